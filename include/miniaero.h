@@ -1,0 +1,171 @@
+/*
+ * miniaero.h -- C ABI of the MI355X-native miniAero explicit RK4 residual path.
+ *
+ * The library (miniaero_amd/libminiaero_amd.so) replaces the data-parallel hot
+ * path that TimeSolverExplicitRK4<Device>::Solve drives in the reference
+ * (kokkos/TimeSolverExplicitRK4.h:160-539): face fluxes (Roe_Flux.h,
+ * Viscous_Flux.h), the face-to-cell residual (Flux.h), Green-Gauss gradients
+ * (GreenGauss.h), stencil min/max and limiters (StencilLimiter.h,
+ * VenkatLimiter.h, VanAlbadaLimiter.h), boundary-face fluxes (Extrapolate_BC.h,
+ * Tangent_BC.h, Inflow_BC.h, NoSlip_BC.h) and the RK4 stage update.  Its host
+ * side keeps the reference's inputs: the nine-value miniaero.inp options
+ * (Options.h:47-101) and the MeshData face/cell lists that
+ * Parallel3DMesh::fillMeshData produces (MeshData.h:42-58, Faces.h:42-71,
+ * Cells.h:41-76).
+ *
+ * Conventions: plain pointers and sizes, no C++ types.  Every entry point
+ * returns ma_status (0 = success); on failure ma_last_error() describes the
+ * cause (thread-local).  Host arrays are row-major, as the reference's
+ * LayoutRight views: a (n x 3) array holds x0 y0 z0 x1 y1 z1 ...
+ * A solver handle is not thread-safe; distinct handles are.
+ */
+#ifndef MINIAERO_H
+#define MINIAERO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int ma_status;
+enum {
+  MA_OK = 0,
+  MA_ERR_INVALID = 1,      /* bad argument or inconsistent mesh */
+  MA_ERR_UNSUPPORTED = 2,  /* valid input outside what this build handles (e.g. ghosts) */
+  MA_ERR_NOMEM = 3,        /* host or device allocation failed */
+  MA_ERR_DEVICE = 4,       /* HIP runtime error, or no GPU */
+  MA_ERR_NONFINITE = 5     /* NaN/Inf found by ma_solver_check_finite */
+};
+
+/* Boundary-condition kinds: the names of MeshData::boundary_faces
+ * (Parallel3DMesh.h:382-396).  Their numeric order is the order in which the
+ * reference applies the boundary fluxes (TimeSolverExplicitRK4.h:428-475). */
+enum { MA_BC_EXTRAPOLATE = 1, MA_BC_TANGENT = 2, MA_BC_NOSLIP = 3, MA_BC_INFLOW = 4 };
+
+/* Side order of the generated box's boundary groups (Parallel3DMesh.h:382-396). */
+enum { MA_SIDE_BOTTOM = 0, MA_SIDE_TOP, MA_SIDE_FRONT, MA_SIDE_BACK, MA_SIDE_RIGHT, MA_SIDE_LEFT };
+
+enum { MA_LIMITER_VENKAT = 0, MA_LIMITER_VANALBADA = 1 };
+enum { MA_IC_REFERENCE = 0, MA_IC_SOD = 1, MA_IC_PERTURBED_INFLOW = 2 };
+
+/* Options.h:47-57 plus config-mode extensions.  A zeroed extension block
+ * reproduces the reference exactly. */
+typedef struct ma_options {
+  int problem_type;          /* 0 Sod, 1 viscous flat plate, 2 inviscid ramp */
+  double lx, ly, lz, angle;  /* domain size; ramp angle in degrees */
+  int nx, ny, nz;            /* cells per direction */
+  int ntimesteps;
+  double dt;
+  int output_results;
+  int output_frequency;
+  int second_order_space;
+  int viscous;
+  /* ---- extensions (no reference counterpart) ---- */
+  int bc_kind[6];            /* per MA_SIDE_*: 0 = reference assignment, else MA_BC_* */
+  double y_stretch;          /* geometric y-spacing ratio away from y=0; 0 or 1 = uniform */
+  int limiter;               /* MA_LIMITER_* */
+  int ic_kind;               /* MA_IC_* */
+  double ic[5];              /* MA_IC_SOD: rhoL, pL, rhoR, pR, x0 */
+  int use_inflow;            /* 1: inflow[] replaces the hard-coded inflow state */
+  double inflow[5];          /* conserved inflow / freestream state */
+  double prandtl;            /* 0 = 0.71 (GasModel.h:103) */
+  double cfl;                /* > 0: dt = cfl * min_c vol_c / sum_f(|u.n| + c|n|), per step */
+  double perturb;            /* MA_IC_PERTURBED_INFLOW: relative amplitude */
+  unsigned long long seed;   /* MA_IC_PERTURBED_INFLOW: generator key */
+} ma_options;
+
+/* One Faces<Device> (Faces.h:42-71), as host arrays. */
+typedef struct ma_faces {
+  int nfaces;
+  const double* coordinates;    /* nfaces x 3, face centroid */
+  const double* face_normal;    /* nfaces x 3, area-weighted normal (left -> right) */
+  const double* face_tangent;   /* nfaces x 3 */
+  const double* face_binormal;  /* nfaces x 3 */
+  const int* face_cell_conn;    /* nfaces x 2: left cell, right cell (-1 on boundary faces) */
+  const int* cell_flux_index;   /* nfaces x 2, local face slot in each cell; may be NULL */
+  const int* permute_vector;    /* nfaces, processing order; NULL = stable sort by left cell */
+} ma_faces;
+
+/* One std::pair<std::string, Faces<Device>> of MeshData::boundary_faces. */
+typedef struct ma_bc_group {
+  int kind;                     /* MA_BC_* (the reference's string name) */
+  ma_faces faces;
+} ma_bc_group;
+
+/* MeshData<Device> (MeshData.h:42-58) + Cells<Device> (Cells.h:41-76), one rank. */
+typedef struct ma_mesh_data {
+  int num_owned_cells;
+  int num_ghosts;                   /* must be 0: one GPU, no halo */
+  const double* cell_coordinates;   /* num_owned_cells x 3 */
+  const double* cell_volumes;       /* num_owned_cells */
+  ma_faces internal_faces;
+  int num_boundary_groups;
+  const ma_bc_group* boundary_faces;
+} ma_mesh_data;
+
+typedef struct ma_mesh ma_mesh;
+typedef struct ma_solver ma_solver;
+
+/* Human-readable description of the last failure on this thread. */
+const char* ma_last_error(void);
+
+/* Reads the nine values of miniaero.inp (Options::read_options_file,
+ * Options.h:73-101) from `path` into *opt; extension fields are zeroed.
+ * Unlike the reference, a missing or short file is an error. */
+ma_status ma_options_read(const char* path, ma_options* opt);
+
+/* ---- mesh (replaces Parallel3DMesh + fillMeshData, Parallel3DMesh.h:56-449,
+ *      Parallel3DMesh.C:36-303, MeshProcessor.C:39-229, Face.C:37-98,
+ *      ElementTopoHexa8.C:134-148) ---- */
+
+/* Generates the reference's one-rank box mesh in closed form: identical
+ * cells, faces, face order (including the std::random_shuffle of internal
+ * faces, which re-seeds the C library rand() with 1) and boundary groups. */
+ma_status ma_mesh_generate(const ma_options* opt, ma_mesh** out);
+/* Fills a view of the generated mesh; pointers stay valid until destroy. */
+ma_status ma_mesh_view(const ma_mesh* mesh, ma_mesh_data* view);
+void ma_mesh_destroy(ma_mesh* mesh);
+
+/* ---- solver (replaces TimeSolverExplicitRK4<Device>, TimeSolverExplicitRK4.h:160-539) ---- */
+
+/* TimeSolverExplicitRK4(MeshData&, const Options&) (TimeSolverExplicitRK4.h:164,179-196):
+ * copies the mesh to the GPU and builds the per-cell face lists.  Picks the
+ * current HIP device. */
+ma_status ma_solver_create(const ma_mesh_data* mesh, const ma_options* opt, ma_solver** out);
+/* Initial condition + copy (TimeSolverExplicitRK4.h:324-336), time = 0. */
+ma_status ma_solver_initialize(ma_solver* s);
+/* Replaces the state (num_owned_cells x 5 conserved values); residual = 0. */
+ma_status ma_solver_set_state(ma_solver* s, const double* U);
+/* Runs nsteps RK4 steps (the body of the time loop, TimeSolverExplicitRK4.h:340-490).
+ * Asynchronous on the solver's stream. */
+ma_status ma_solver_step(ma_solver* s, int nsteps);
+/* Blocks until the solver's stream is idle. */
+ma_status ma_solver_synchronize(ma_solver* s);
+/* Copies the current solution (sol_n) to host, num_owned_cells x 5. */
+ma_status ma_solver_get_state(ma_solver* s, double* U);
+/* Residual of the last stage, num_owned_cells x 5. */
+ma_status ma_solver_get_residual(ma_solver* s, double* R);
+/* Gradients (n x 5 x 3) and limiters (n x 5) of the last stage; either may be NULL. */
+ma_status ma_solver_get_gradients(ma_solver* s, double* grad, double* lim);
+/* Simulated time and the dt of the last step. */
+ma_status ma_solver_get_time(ma_solver* s, double* time, double* last_dt);
+/* Sets *nonfinite to the number of non-finite values in the state. */
+ma_status ma_solver_check_finite(ma_solver* s, long long* nonfinite);
+/* The HIP stream (hipStream_t) all of the solver's work is queued on. */
+ma_status ma_solver_stream(ma_solver* s, void** stream);
+void ma_solver_destroy(ma_solver* s);
+
+/* ---- measurement ---- */
+
+/* Per-kernel timing: runs nsteps with a HIP event pair around every launch on
+ * the solver's stream and reports, for each kernel name, the number of
+ * launches and the mean duration in milliseconds.  Returns the number of
+ * kernel names written (at most max). names[i] points to static strings. */
+int ma_solver_profile(ma_solver* s, int nsteps, const char** names, int* launches,
+                      double* mean_ms, int max);
+/* Sizes: counts[0] cells, [1] internal faces, [2] boundary faces, [3] device bytes held. */
+ma_status ma_solver_sizes(ma_solver* s, long long* counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,65 @@
+"""Named workloads.
+
+The reference runs only its three hard-coded problems (problem_type 0/1/2,
+Parallel3DMesh.h:382-396, TimeSolverExplicitRK4.h:218-223).  The benchmark
+workloads below use the config-mode extensions of the options struct to set
+the freestream from a Mach and Reynolds number, stretch the mesh toward the
+wall, choose boundary kinds per side and pick dt from a CFL number.
+"""
+from __future__ import annotations
+
+import math
+
+from .options import Options
+
+RGAS = 287.05    # GasModel.h:39
+GAMMA = 1.4      # GasModel.h:51
+
+
+def sutherland(T: float) -> float:
+    """GasModel.h:92-98."""
+    return 1.458e-6 * T * math.sqrt(T) / (T + 110.4)
+
+
+def freestream_state(mach: float, T: float, re_per_m: float, angle_deg: float = 0.0,
+                     length: float = 1.0) -> tuple:
+    """Conserved state (rho, rho u, rho v, rho w, rho E) of a freestream at
+    the given Mach number, temperature and unit Reynolds number; the flow
+    direction is turned by angle_deg in the x-y plane."""
+    c = math.sqrt(GAMMA * RGAS * T)
+    speed = mach * c
+    rho = re_per_m * sutherland(T) / (speed * length) if re_per_m > 0 else 1.0
+    a = math.radians(angle_deg)
+    u, v = speed * math.cos(a), speed * math.sin(a)
+    E = rho * (RGAS / (GAMMA - 1.0)) * T + 0.5 * rho * speed * speed
+    return (rho, rho * u, rho * v, 0.0, E)
+
+
+def sod_c1(**kw) -> Options:
+    """3D Sod shock tube on [0,1]x[0,0.25]x[0,0.25], 64x16x16, first order,
+    Extrapolate on every side, rho/p = 1/1 left, 0.125/0.1 right, dt 1e-4."""
+    o = Options(problem_type=0, lx=1.0, ly=0.25, lz=0.25, nx=64, ny=16, nz=16, ntimesteps=100,
+                dt=1e-4, second_order_space=0, viscous=0, bc_kind=(1, 1, 1, 1, 1, 1), ic_kind=1,
+                ic=(1.0, 1.0, 0.125, 0.1, 0.5))
+    return o.replace(**kw)
+
+
+def flat_plate_c2(**kw) -> Options:
+    """Viscous flat plate, 256x128x64 on [0,1]x[0,0.05]x[0,0.025], y-spacing
+    growing by 1.05 away from the wall; M 0.3, T 300 K, Re 1e5 per metre,
+    Pr 0.72; Inflow -x, Extrapolate +x, NoSlip -y, Tangent +y and +-z; Roe +
+    viscous, Green-Gauss, Venkat, second order; CFL 0.5, 200 steps."""
+    o = Options(problem_type=1, lx=1.0, ly=0.05, lz=0.025, nx=256, ny=128, nz=64, ntimesteps=200,
+                dt=0.0, second_order_space=1, viscous=1, y_stretch=1.05,
+                bc_kind=(3, 2, 2, 2, 1, 4), use_inflow=1,
+                inflow=freestream_state(0.3, 300.0, 1.0e5), prandtl=0.72, cfl=0.5)
+    return o.replace(**kw)
+
+
+def flat_plate_c3(**kw) -> Options:
+    """The C2 physics on 512x256x64, started from the freestream with a seeded
+    +-1% perturbation of density and velocity, 500 steps."""
+    return flat_plate_c2(nx=512, ny=256, nz=64, ntimesteps=500, ic_kind=2, perturb=0.01, seed=3).replace(**kw)
+
+
+WORKLOADS = {"sod_c1": sod_c1, "flat_plate_c2": flat_plate_c2, "flat_plate_c3": flat_plate_c3}

@@ -1,0 +1,1012 @@
+// solver.hip -- MI355X (gfx950) kernels and C ABI of the miniAero RK4 path.
+//
+// Replaces the kernel sequence that TimeSolverExplicitRK4::Solve launches per
+// RK stage (TimeSolverExplicitRK4.h:352-486): update, zero_cell_flux,
+// GreenGauss (3 kernels + per-group launches), StencilLimiter min/max and
+// limiter (6 kernels + per-group launches), compute_face_flux, one launch per
+// boundary group and kind, apply_cell_flux and the RK accumulate -- about 37
+// launches and fences per second-order stage -- with three cell-centric
+// kernels and no atomics:
+//
+//   k_prim   U_s = U_n + alpha_k R;  V = primitives(U_s)           (per cell)
+//   k_grad   Green-Gauss gradient, stencil min/max and the limiter  (per cell)
+//            of every cell, from V of the cell and its face neighbours
+//   k_flux   every face flux of the cell (Roe [+ reconstruction]    (per cell)
+//            [+ viscous]), boundary fluxes, residual R = dt/V sum,
+//            RK accumulate U_acc += beta_k R
+//
+// Determinism and parity.  The reference scatters face results into cells
+// with atomics; with one thread the additions land in a fixed order: internal
+// faces in the permuted order (stable by left cell), then boundary faces.  At
+// setup every cell gets its face list in exactly that order (slot table), and
+// each kernel sums its cell's contributions in slot order.  An internal face
+// flux is evaluated by both of its cells -- bitwise the same value, since it is
+// a pure function of the same inputs -- so no cell ever waits on another and
+// the result does not depend on scheduling.  With -ffp-contract=off the FP64
+// results equal those of a one-thread CPU run of the reference.
+//
+// Layout in HBM (structure of arrays, one plane per component, so a
+// wavefront's 64 lanes read 64 consecutive doubles):
+//   cells   vol[nc], xc[3][nc], U[2][5][nc] (U_n / U_acc ping-pong), R[5][nc],
+//           V[5][nc], grad[15][nc], phi[5][nc], slot[6][nc] (int32)
+//   faces   n[3][nf], t[3][nf], b[3][nf], xf[3][nf], left[nf], right[nf],
+//           kind[nf] (int8), wall[nf]; internal faces stored in processing
+//           (left-cell) order, then the boundary groups
+// Entry point declarations and the reference interfaces they replace are in
+// include/miniaero.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/miniaero.h"
+#include "common.hpp"
+#include "physics.hpp"
+
+namespace {
+
+constexpr int kMaxSlots = 6;  // hex8 cells (ElementTopoHexa8); checked at setup
+constexpr int kBlock = 256;
+
+struct DevMesh {
+  int nc;    // owned cells
+  int nif;   // internal faces
+  int nf;    // all faces
+  const double* __restrict__ vol;
+  const double* __restrict__ xc;   // 3 planes of nc
+  const int* __restrict__ slot;    // kMaxSlots planes of nc; entry = face*2 + side, -1 empty
+  const double* __restrict__ fn;   // 3 planes of nf
+  const double* __restrict__ ft;
+  const double* __restrict__ fb;
+  const double* __restrict__ fx;
+  const int* __restrict__ fl;
+  const int* __restrict__ fr;
+  const signed char* __restrict__ fkind;
+  const double* __restrict__ fwall;  // 1 / |x_f - x_c| of boundary faces
+};
+
+struct Inflow {
+  double U[5];
+};
+
+__device__ __forceinline__ void load5(const double* __restrict__ a, int n, int i, double* v) {
+#pragma unroll
+  for (int k = 0; k < 5; ++k) v[k] = a[(size_t)k * n + i];
+}
+
+__device__ __forceinline__ void load3(const double* __restrict__ a, int n, int i, double* v) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) v[k] = a[(size_t)k * n + i];
+}
+
+// ---------------------------------------------------------------- kernels --
+
+// Initial condition: initialize_sod3d / initialize_constant
+// (Initial_Conditions.h:38-133) and the Sod-type / perturbed-inflow
+// extensions, written to U_n.
+struct IcParams {
+  int kind;          // MA_IC_*
+  int problem_type;
+  double midx;       // reference Sod split, lx / 2
+  double ic[5];
+  double inflow[5];
+  double perturb;
+  unsigned long long seed;
+};
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Counter-based uniform draw in [-1, 1) keyed by (seed, cell, k).
+__device__ __forceinline__ double unif(unsigned long long seed, long long cell, int k) {
+  unsigned long long z = mix64(seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(cell * 8 + k + 1));
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_init(DevMesh m, IcParams p, double* __restrict__ U) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.nc) return;
+  const double Rgas = 287.05, gamma = 1.4;
+  const double x = m.xc[c];
+  double u[5];
+  if (p.kind == MA_IC_SOD) {
+    const double Cv = Rgas / (gamma - 1.0);
+    const bool left = x < p.ic[4];
+    const double rho = left ? p.ic[0] : p.ic[2];
+    const double pr = left ? p.ic[1] : p.ic[3];
+    const double T = pr / (rho * Rgas);
+    u[0] = rho;
+    u[1] = u[2] = u[3] = 0.0;
+    u[4] = rho * (Cv * T);
+  } else if (p.kind == MA_IC_PERTURBED_INFLOW) {
+    double V[5];
+    ma::primitives(p.inflow, V);
+    const double speed = sqrt(V[1] * V[1] + V[2] * V[2] + V[3] * V[3]);
+    const double pr = V[0] * Rgas * V[4];
+    const double rho = V[0] * (1.0 + p.perturb * unif(p.seed, c, 0));
+    const double uu = V[1] + p.perturb * speed * unif(p.seed, c, 1);
+    const double vv = V[2] + p.perturb * speed * unif(p.seed, c, 2);
+    const double ww = V[3] + p.perturb * speed * unif(p.seed, c, 3);
+    const double T = pr / (rho * Rgas);
+    const double Cv = Rgas / (gamma - 1.0);
+    u[0] = rho;
+    u[1] = rho * uu;
+    u[2] = rho * vv;
+    u[3] = rho * ww;
+    u[4] = rho * (Cv * T) + 0.5 * rho * (uu * uu + vv * vv + ww * ww);
+  } else if (p.problem_type == 0) {
+    const double Cv = Rgas / (gamma - 1.0);
+    const double P1 = 68947.57, T1 = 288.889, P2 = 6894.757, T2 = 231.11;
+    const double d1 = P1 / (Rgas * T1), e1 = d1 * (Cv * T1);
+    const double d2 = P2 / (Rgas * T2), e2 = d2 * (Cv * T2);
+    u[0] = x < p.midx ? d1 : d2;
+    u[1] = u[2] = u[3] = 0.0;
+    u[4] = x < p.midx ? e1 : e2;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) u[k] = p.inflow[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) U[(size_t)k * m.nc + c] = u[k];
+}
+
+// Stage state and its primitives: update<Device> with alpha_k
+// (TimeSolverExplicitRK4.h:106-128, 355-357) fused with ComputePrimitives.
+__global__ __launch_bounds__(kBlock) void k_prim(int nc, const double* __restrict__ Un,
+                                                 const double* __restrict__ R, double alpha,
+                                                 double* __restrict__ V) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  double u[5], v[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) u[k] = Un[(size_t)k * nc + c] + alpha * R[(size_t)k * nc + c];
+  ma::primitives(u, v);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) V[(size_t)k * nc + c] = v[k];
+}
+
+// Green-Gauss gradient (GreenGauss.h:51-270), then, for second order, the
+// stencil min/max (StencilLimiter.h:56-282) and the limiter
+// (StencilLimiter.h:288-500) of one cell, all from the cell's slot list.
+template <bool kSecondOrder, class Limiter>
+__global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const double* __restrict__ V,
+                                                 double* __restrict__ grad,
+                                                 double* __restrict__ phi) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.nc) return;
+  const int nc = m.nc, nf = m.nf;
+  double vc[5];
+  load5(V, nc, c, vc);
+  const double vol = m.vol[c];
+  int ent[kMaxSlots];
+#pragma unroll
+  for (int s = 0; s < kMaxSlots; ++s) ent[s] = m.slot[(size_t)s * nc + c];
+
+  double g[5][3];
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) g[k][d] = 0.0;
+  double mn[5], mx[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    mn[k] = 1.0e300;
+    mx[k] = -1.0e300;
+  }
+
+#pragma unroll
+  for (int s = 0; s < kMaxSlots; ++s) {
+    const int e = ent[s];
+    if (e < 0) break;
+    const int f = e >> 1;
+    double n[3];
+    load3(m.fn, nf, f, n);
+    if (f < m.nif) {
+      const int side = e & 1;
+      const int nb = side ? m.fl[f] : m.fr[f];
+      double vn[5];
+      load5(V, nc, nb, vn);
+      const double* vl = side ? vn : vc;
+      const double* vr = side ? vc : vn;
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          const double gg = 0.5 * (vl[k] + vr[k]) * n[d];
+          g[k][d] += side ? -gg / vol : gg / vol;
+        }
+      if (kSecondOrder) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          const double fmin = vr[k] < vl[k] ? vr[k] : vl[k];
+          const double fmax = vr[k] > vl[k] ? vr[k] : vl[k];
+          mn[k] = ma::smin(mn[k], fmin);
+          mx[k] = ma::smax(mx[k], fmax);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 5; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) g[k][d] += vc[k] * n[d] / vol;
+      if (kSecondOrder) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          mn[k] = ma::smin(mn[k], vc[k]);
+          mx[k] = ma::smax(mx[k], vc[k]);
+        }
+      }
+    }
+  }
+  // green_gauss_gradient_sum: gradient = 0; gradient += slot (GreenGauss.h:244-269)
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double x = 0;
+      x += g[k][d];
+      g[k][d] = x;
+      grad[(size_t)(k * 3 + d) * nc + c] = x;
+    }
+  if (!kSecondOrder) return;
+
+  double xcc[3];
+  load3(m.xc, nc, c, xcc);
+  double lim[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) lim[k] = 1.0;
+#pragma unroll
+  for (int s = 0; s < kMaxSlots; ++s) {
+    const int e = ent[s];
+    if (e < 0) break;
+    const int f = e >> 1;
+    double disp[3], dist = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      disp[d] = m.fx[(size_t)d * nf + f] - xcc[d];
+      dist += disp[d] * disp[d];
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      double du = 0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) du += disp[d] * g[k][d];
+      const double dumax = mx[k] - vc[k];
+      const double dumin = mn[k] - vc[k];
+      lim[k] = ma::smin(lim[k], Limiter::limit(dumax, dumin, du, dist));
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) phi[(size_t)k * nc + c] = lim[k];
+}
+
+// Reconstructed (second order) primitive state of cell `c` at face centroid xf:
+// compute_face_flux's extrapolation, Flux.h:109-132.
+__device__ __forceinline__ void reconstruct(const DevMesh& m, const double* __restrict__ grad,
+                                            const double* __restrict__ phi, int c,
+                                            const double* xf, double* v) {
+  const int nc = m.nc;
+  double dx[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) dx[d] = xf[d] - m.xc[(size_t)d * nc + c];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    double t = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) t += dx[d] * grad[(size_t)(k * 3 + d) * nc + c];
+    v[k] += t * phi[(size_t)k * nc + c];
+  }
+}
+
+// All face fluxes of one cell, residual and RK accumulate:
+// compute_face_flux (Flux.h:52-182), the four boundary functors
+// (Extrapolate_BC.h:45-102, Tangent_BC.h:47-123, NoSlip_BC.h:47-157,
+// Inflow_BC.h:45-111) in the reference's kind order, apply_cell_flux
+// (Flux.h:188-229) and update<Device> with beta_k (TimeSolverExplicitRK4.h:483-485).
+template <bool kSecondOrder, class Visc>
+__global__ __launch_bounds__(kBlock) void k_flux(DevMesh m, const double* __restrict__ V,
+                                                 const double* __restrict__ grad,
+                                                 const double* __restrict__ phi,
+                                                 const double* Uacc_in,  // may alias Uacc_out
+                                                 double* Uacc_out,
+                                                 double* __restrict__ R,
+                                                 const double* __restrict__ dt_dev, double beta,
+                                                 Visc visc, Inflow inflow) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.nc) return;
+  const int nc = m.nc, nf = m.nf;
+  const ma::RoeFlux roe;
+  double vc[5];
+  load5(V, nc, c, vc);
+  int ent[kMaxSlots];
+#pragma unroll
+  for (int s = 0; s < kMaxSlots; ++s) ent[s] = m.slot[(size_t)s * nc + c];
+
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+
+#pragma unroll
+  for (int s = 0; s < kMaxSlots; ++s) {
+    const int e = ent[s];
+    if (e < 0 || (e >> 1) >= m.nif) break;
+    const int f = e >> 1, side = e & 1;
+    const int L = m.fl[f], Rr = m.fr[f];
+    double vl[5], vr[5];
+    load5(V, nc, L, vl);
+    load5(V, nc, Rr, vr);
+    double n[3], t[3], b[3];
+    load3(m.fn, nf, f, n);
+    load3(m.ft, nf, f, t);
+    load3(m.fb, nf, f, b);
+    if (kSecondOrder) {
+      double xf[3];
+      load3(m.fx, nf, f, xf);
+      reconstruct(m, grad, phi, L, xf, vl);
+      reconstruct(m, grad, phi, Rr, xf, vr);
+    }
+    double F[5];
+    roe.compute_flux(vl, vr, F, n, t, b);
+    if (Visc::isViscous) {
+      double vf[5], gf[5][3], fv[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        vf[k] = 0.5 * (vl[k] + vr[k]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          gf[k][d] = 0.5 * (grad[(size_t)(k * 3 + d) * nc + L] + grad[(size_t)(k * 3 + d) * nc + Rr]);
+      }
+      visc.compute_flux(gf, vf, n, fv);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) F[k] -= fv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) acc[k] += side ? F[k] : -F[k];
+  }
+
+  // Boundary faces: Extrapolate, Tangent, NoSlip, Inflow, each in group order.
+  bool any_bc = false;
+#pragma unroll
+  for (int s = 0; s < kMaxSlots; ++s) any_bc |= ent[s] >= 0 && (ent[s] >> 1) >= m.nif;
+  if (any_bc) {
+    for (int kind = MA_BC_EXTRAPOLATE; kind <= MA_BC_INFLOW; ++kind) {
+      for (int s = 0; s < kMaxSlots; ++s) {
+        const int e = ent[s];
+        if (e < 0) break;
+        const int f = e >> 1;
+        if (f < m.nif || m.fkind[f] != kind) continue;
+        double n[3], t[3], b[3], vr[5], F[5];
+        load3(m.fn, nf, f, n);
+        load3(m.ft, nf, f, t);
+        load3(m.fb, nf, f, b);
+        if (kind == MA_BC_EXTRAPOLATE) {
+          roe.compute_flux(vc, vc, F, n, t, b);
+#pragma unroll
+          for (int k = 0; k < 5; ++k) acc[k] += -F[k];
+        } else if (kind == MA_BC_INFLOW) {
+          ma::primitives(inflow.U, vr);
+          roe.compute_flux(vc, vr, F, n, t, b);
+#pragma unroll
+          for (int k = 0; k < 5; ++k) acc[k] += -F[k];
+        } else {
+          double an = 0;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) an += n[d] * n[d];
+          an = sqrt(an);
+          double ub = 0.0;
+          ub += vc[1] * n[0] / an;
+          ub += vc[2] * n[1] / an;
+          ub += vc[3] * n[2] / an;
+          vr[0] = vc[0];
+          vr[1] = vc[1] - 2 * ub * n[0] / an;
+          vr[2] = vc[2] - 2 * ub * n[1] / an;
+          vr[3] = vc[3] - 2 * ub * n[2] / an;
+          vr[4] = vc[4];
+          roe.compute_flux(vc, vr, F, n, t, b);
+          if (kind == MA_BC_TANGENT) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) acc[k] += -F[k];
+          } else {
+            // no-slip wall: newtonian viscous flux from the one-sided wall
+            // gradient, applied even in inviscid runs (TimeSolverExplicitRK4.h:457-460)
+            const double vf[5] = {vc[0], 0.0, 0.0, 0.0, vc[4]};
+            const double inv = m.fwall[f];
+            double gf[5][3], fv[5], un[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) un[d] = n[d] / an;
+#pragma unroll
+            for (int k = 0; k < 5; ++k)
+#pragma unroll
+              for (int d = 0; d < 3; ++d) gf[k][d] = (vf[k] - vc[k]) * un[d] * inv;
+            ma::NewtonianViscousFlux wall{visc.prandtl};
+            wall.compute_flux(gf, vf, n, fv);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) acc[k] += -F[k] + fv[k];
+          }
+        }
+      }
+    }
+  }
+
+  const double dt = *dt_dev;
+  const double vol = m.vol[c];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    double r = 0.0;
+    r = r + dt / vol * acc[k];
+    R[(size_t)k * nc + c] = r;
+    Uacc_out[(size_t)k * nc + c] = Uacc_in[(size_t)k * nc + c] + beta * r;
+  }
+}
+
+// Stable time step for the CFL extension: per cell vol / sum_f (|u.n| + c|n|)
+// over the slot list, reduced to the minimum per block.
+__global__ __launch_bounds__(kBlock) void k_dt_partial(DevMesh m, const double* __restrict__ Un,
+                                                       double* __restrict__ partial) {
+  __shared__ double red[kBlock / 64];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double dtc = 1e300;
+  if (c < m.nc) {
+    const int nc = m.nc, nf = m.nf;
+    double u[5], v[5];
+    load5(Un, nc, c, u);
+    ma::primitives(u, v);
+    const double snd = sqrt(1.4 * (v[0] * ma::kRgas * v[4]) / v[0]);
+    double s = 0.0;
+    for (int q = 0; q < kMaxSlots; ++q) {
+      const int e = m.slot[(size_t)q * nc + c];
+      if (e < 0) break;
+      const int f = e >> 1;
+      double n[3];
+      load3(m.fn, nf, f, n);
+      const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      s = s + (fabs(v[1] * n[0] + v[2] * n[1] + v[3] * n[2]) + snd * nn);
+    }
+    dtc = m.vol[c] / s;
+  }
+  for (int off = 32; off > 0; off >>= 1) dtc = ma::smin(dtc, __shfl_down(dtc, off, 64));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = dtc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = red[0];
+    for (int w = 1; w < kBlock / 64; ++w) x = ma::smin(x, red[w]);
+    partial[blockIdx.x] = x;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dt_final(const double* __restrict__ partial, int np,
+                                                     double cfl, double* __restrict__ dt_dev,
+                                                     double* __restrict__ time_dev) {
+  __shared__ double red[kBlock / 64];
+  double x = 1e300;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) x = ma::smin(x, partial[i]);
+  for (int off = 32; off > 0; off >>= 1) x = ma::smin(x, __shfl_down(x, off, 64));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double y = red[0];
+    for (int w = 1; w < kBlock / 64; ++w) y = ma::smin(y, red[w]);
+    const double dt = cfl * y;
+    *dt_dev = dt;
+    *time_dev = *time_dev + dt;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_count_nonfinite(int n, const double* __restrict__ a,
+                                                            unsigned long long* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool bad = i < n && !isfinite(a[i]);
+  const unsigned long long mask = __ballot(bad);
+  if ((threadIdx.x & 63) == 0 && mask) atomicAdd(out, (unsigned long long)__popcll(mask));
+}
+
+// ------------------------------------------------------------- host side --
+
+#define MA_HIP(call)                                                                      \
+  do {                                                                                    \
+    hipError_t err_ = (call);                                                             \
+    if (err_ != hipSuccess)                                                               \
+      return ma::fail(MA_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(err_)); \
+  } while (0)
+
+struct Timing {
+  bool on = false;
+  struct Rec {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+};
+
+}  // namespace
+
+struct ma_solver {
+  ma_options opt;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DevMesh m{};
+  std::vector<void*> allocs;
+  size_t bytes = 0;
+  double *U[2] = {nullptr, nullptr}, *R = nullptr, *V = nullptr, *grad = nullptr, *phi = nullptr;
+  double *dt_dev = nullptr, *time_dev = nullptr, *partial = nullptr;
+  unsigned long long* counter = nullptr;
+  int cur = 0;  // U[cur] holds sol_n
+  double time = 0.0, last_dt = 0.0;
+  Inflow inflow{};
+  Timing timing;
+
+  template <class T>
+  ma_status alloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess)
+      return ma::fail(MA_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocs.push_back(*p);
+    bytes += n * sizeof(T);
+    return MA_OK;
+  }
+
+  // Launch wrapper: records a HIP event pair around the launch when profiling.
+  template <class F>
+  void launch(const char* name, F&& f) {
+    if (timing.on) {
+      Timing::Rec r{name, nullptr, nullptr};
+      (void)hipEventCreate(&r.a);
+      (void)hipEventCreate(&r.b);
+      (void)hipEventRecord(r.a, stream);
+      f();
+      (void)hipEventRecord(r.b, stream);
+      timing.recs.push_back(r);
+    } else {
+      f();
+    }
+  }
+};
+
+namespace {
+
+inline int nblocks(int n) { return (n + kBlock - 1) / kBlock; }
+
+ma_status check_faces(const ma_faces& F, int nc, bool internal, const char* what) {
+  if (F.nfaces < 0) return ma::fail(MA_ERR_INVALID, std::string(what) + ": negative nfaces");
+  if (F.nfaces == 0) return MA_OK;
+  if (!F.coordinates || !F.face_normal || !F.face_tangent || !F.face_binormal || !F.face_cell_conn)
+    return ma::fail(MA_ERR_INVALID, std::string(what) + ": null face array");
+  for (int i = 0; i < F.nfaces; ++i) {
+    const int L = F.face_cell_conn[2 * i], R = F.face_cell_conn[2 * i + 1];
+    if (L < 0 || L >= nc) return ma::fail(MA_ERR_INVALID, std::string(what) + ": left cell out of range");
+    if (internal && (R < 0 || R >= nc || R == L))
+      return ma::fail(MA_ERR_INVALID, std::string(what) + ": right cell out of range");
+  }
+  if (F.permute_vector) {
+    std::vector<char> seen(F.nfaces, 0);
+    for (int i = 0; i < F.nfaces; ++i) {
+      const int p = F.permute_vector[i];
+      if (p < 0 || p >= F.nfaces || seen[p])
+        return ma::fail(MA_ERR_INVALID, std::string(what) + ": permute_vector is not a permutation");
+      seen[p] = 1;
+    }
+  }
+  return MA_OK;
+}
+
+// Processing order of a face list: the given permutation, or a stable sort by
+// left cell (what Faces.h:128-144 builds).
+std::vector<int> processing_order(const ma_faces& F, int nc) {
+  std::vector<int> perm(F.nfaces);
+  if (F.permute_vector) {
+    std::copy(F.permute_vector, F.permute_vector + F.nfaces, perm.begin());
+    return perm;
+  }
+  std::vector<int> start((size_t)nc + 1, 0);
+  for (int f = 0; f < F.nfaces; ++f) start[F.face_cell_conn[2 * f] + 1]++;
+  for (int c = 0; c < nc; ++c) start[c + 1] += start[c];
+  for (int f = 0; f < F.nfaces; ++f) perm[start[F.face_cell_conn[2 * f]]++] = f;
+  return perm;
+}
+
+template <class T>
+ma_status upload(ma_solver* s, T** dst, const std::vector<T>& h) {
+  ma_status st = s->alloc(dst, h.size());
+  if (st) return st;
+  MA_HIP(hipMemcpy(*dst, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return MA_OK;
+}
+
+ma_status build(ma_solver* s, const ma_mesh_data* md) {
+  const int nc = md->num_owned_cells;
+  const ma_faces& I = md->internal_faces;
+  ma_status st = check_faces(I, nc, true, "internal_faces");
+  if (st) return st;
+  long long nbf = 0;
+  for (int g = 0; g < md->num_boundary_groups; ++g) {
+    const ma_bc_group& G = md->boundary_faces[g];
+    if (G.kind < MA_BC_EXTRAPOLATE || G.kind > MA_BC_INFLOW)
+      return ma::fail(MA_ERR_INVALID, "boundary_faces: unknown kind");
+    st = check_faces(G.faces, nc, false, "boundary_faces");
+    if (st) return st;
+    nbf += G.faces.nfaces;
+  }
+  if ((long long)I.nfaces + nbf > (1ll << 30))
+    return ma::fail(MA_ERR_UNSUPPORTED, "more than 2^30 faces");
+  const int nif = I.nfaces, nf = nif + (int)nbf;
+
+  // Device face order: internal faces in processing order, then every
+  // boundary group in order.
+  const std::vector<int> perm = processing_order(I, nc);
+  std::vector<double> fn((size_t)3 * nf), ft((size_t)3 * nf), fb((size_t)3 * nf), fx((size_t)3 * nf);
+  std::vector<int> fl(nf), fr(nf);
+  std::vector<signed char> fkind(nf, 0);
+  std::vector<double> fwall(nf, 0.0);
+  std::vector<int> slot((size_t)kMaxSlots * nc, -1);
+  std::vector<unsigned char> used(nc, 0);
+  auto put = [&](int f, const ma_faces& F, int i) {
+    for (int d = 0; d < 3; ++d) {
+      fn[(size_t)d * nf + f] = F.face_normal[3 * i + d];
+      ft[(size_t)d * nf + f] = F.face_tangent[3 * i + d];
+      fb[(size_t)d * nf + f] = F.face_binormal[3 * i + d];
+      fx[(size_t)d * nf + f] = F.coordinates[3 * i + d];
+    }
+  };
+  auto add_slot = [&](int c, int entry) -> bool {
+    if (used[c] >= kMaxSlots) return false;
+    slot[(size_t)used[c] * nc + c] = entry;
+    used[c]++;
+    return true;
+  };
+  for (int ii = 0; ii < nif; ++ii) {
+    const int i = perm[ii];
+    put(ii, I, i);
+    fl[ii] = I.face_cell_conn[2 * i];
+    fr[ii] = I.face_cell_conn[2 * i + 1];
+    if (!add_slot(fl[ii], ii * 2) || !add_slot(fr[ii], ii * 2 + 1))
+      return ma::fail(MA_ERR_UNSUPPORTED, "a cell has more than 6 faces (hex8 meshes only)");
+  }
+  int f = nif;
+  for (int g = 0; g < md->num_boundary_groups; ++g) {
+    const ma_bc_group& G = md->boundary_faces[g];
+    for (int i = 0; i < G.faces.nfaces; ++i, ++f) {
+      put(f, G.faces, i);
+      const int c = G.faces.face_cell_conn[2 * i];
+      fl[f] = c;
+      fr[f] = -1;
+      fkind[f] = (signed char)G.kind;
+      // NoSlip_BC.h:124-129: 1 / sqrt(sum_d pow(x_f - x_c, 2))
+      double dist = 0;
+      for (int d = 0; d < 3; ++d)
+        dist += std::pow(G.faces.coordinates[3 * i + d] - md->cell_coordinates[3 * c + d], 2);
+      fwall[f] = 1.0 / std::sqrt(dist);
+      if (!add_slot(c, f * 2))
+        return ma::fail(MA_ERR_UNSUPPORTED, "a cell has more than 6 faces (hex8 meshes only)");
+    }
+  }
+  std::vector<double> vol(md->cell_volumes, md->cell_volumes + nc);
+  std::vector<double> xc((size_t)3 * nc);
+  for (int c = 0; c < nc; ++c)
+    for (int d = 0; d < 3; ++d) xc[(size_t)d * nc + c] = md->cell_coordinates[3 * c + d];
+
+  DevMesh& m = s->m;
+  m.nc = nc;
+  m.nif = nif;
+  m.nf = nf;
+  double *dvol, *dxc, *dfn, *dft, *dfb, *dfx, *dfw;
+  int *dslot, *dfl, *dfr;
+  signed char* dkind;
+  if ((st = upload(s, &dvol, vol)) || (st = upload(s, &dxc, xc)) || (st = upload(s, &dslot, slot)) ||
+      (st = upload(s, &dfn, fn)) || (st = upload(s, &dft, ft)) || (st = upload(s, &dfb, fb)) ||
+      (st = upload(s, &dfx, fx)) || (st = upload(s, &dfl, fl)) || (st = upload(s, &dfr, fr)) ||
+      (st = upload(s, &dkind, fkind)) || (st = upload(s, &dfw, fwall)))
+    return st;
+  m.vol = dvol;
+  m.xc = dxc;
+  m.slot = dslot;
+  m.fn = dfn;
+  m.ft = dft;
+  m.fb = dfb;
+  m.fx = dfx;
+  m.fl = dfl;
+  m.fr = dfr;
+  m.fkind = dkind;
+  m.fwall = dfw;
+
+  const size_t n5 = (size_t)5 * nc;
+  if ((st = s->alloc(&s->U[0], n5)) || (st = s->alloc(&s->U[1], n5)) || (st = s->alloc(&s->R, n5)) ||
+      (st = s->alloc(&s->V, n5)) || (st = s->alloc(&s->grad, 3 * n5)) || (st = s->alloc(&s->phi, n5)) ||
+      (st = s->alloc(&s->dt_dev, 1)) || (st = s->alloc(&s->time_dev, 1)) ||
+      (st = s->alloc(&s->partial, (size_t)nblocks(nc))) || (st = s->alloc(&s->counter, 1)))
+    return st;
+  MA_HIP(hipMemset(s->R, 0, n5 * sizeof(double)));
+  MA_HIP(hipMemset(s->grad, 0, 3 * n5 * sizeof(double)));
+  MA_HIP(hipMemset(s->phi, 0, n5 * sizeof(double)));
+  MA_HIP(hipMemcpy(s->dt_dev, &s->opt.dt, sizeof(double), hipMemcpyHostToDevice));
+  MA_HIP(hipMemset(s->time_dev, 0, sizeof(double)));
+  return MA_OK;
+}
+
+template <bool SO, class Visc, class Lim>
+void run_stage(ma_solver* s, int irk) {
+  // TimeSolverExplicitRK4.h:187-195
+  constexpr double alpha[4] = {0.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};
+  constexpr double beta[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+  const DevMesh& m = s->m;
+  const int nb = nblocks(m.nc);
+  double* Un = s->U[s->cur];
+  double* Uacc = s->U[1 - s->cur];
+  s->launch("k_prim", [&] {
+    hipLaunchKernelGGL(k_prim, dim3(nb), dim3(kBlock), 0, s->stream, m.nc, Un, s->R, alpha[irk], s->V);
+  });
+  if (SO || Visc::isViscous) {
+    s->launch(SO ? "k_grad_limiter" : "k_grad", [&] {
+      hipLaunchKernelGGL((k_grad<SO, Lim>), dim3(nb), dim3(kBlock), 0, s->stream, m, s->V, s->grad, s->phi);
+    });
+  }
+  Visc visc{s->opt.prandtl > 0.0 ? s->opt.prandtl : 0.71};
+  // Stage 0 accumulates from sol_n (sol_np1 == sol_n at the start of a step).
+  const double* acc_in = irk == 0 ? Un : Uacc;
+  s->launch("k_flux", [&] {
+    hipLaunchKernelGGL((k_flux<SO, Visc>), dim3(nb), dim3(kBlock), 0, s->stream, m, s->V, s->grad,
+                       s->phi, acc_in, Uacc, s->R, s->dt_dev, beta[irk], visc, s->inflow);
+  });
+}
+
+template <bool SO, class Visc, class Lim>
+void run_steps(ma_solver* s, int nsteps) {
+  for (int it = 0; it < nsteps; ++it) {
+    if (s->opt.cfl > 0.0) {
+      const int nb = nblocks(s->m.nc);
+      s->launch("k_dt_partial", [&] {
+        hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(kBlock), 0, s->stream, s->m, s->U[s->cur], s->partial);
+      });
+      s->launch("k_dt_final", [&] {
+        hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kBlock), 0, s->stream, s->partial, nb, s->opt.cfl,
+                           s->dt_dev, s->time_dev);
+      });
+    } else {
+      s->time += s->opt.dt;
+      s->last_dt = s->opt.dt;
+    }
+    for (int irk = 0; irk < 4; ++irk) run_stage<SO, Visc, Lim>(s, irk);
+    s->cur = 1 - s->cur;  // U_acc after the last stage is the new sol_n
+  }
+}
+
+template <bool SO, class Visc>
+void dispatch_lim(ma_solver* s, int n) {
+  if (s->opt.limiter == MA_LIMITER_VANALBADA)
+    run_steps<SO, Visc, ma::VanAlbadaLimiter>(s, n);
+  else
+    run_steps<SO, Visc, ma::VenkatLimiter>(s, n);
+}
+
+void dispatch(ma_solver* s, int n) {
+  const bool so = s->opt.second_order_space != 0, visc = s->opt.viscous != 0;
+  if (so && visc) dispatch_lim<true, ma::NewtonianViscousFlux>(s, n);
+  else if (so) dispatch_lim<true, ma::NoViscousFlux>(s, n);
+  else if (visc) dispatch_lim<false, ma::NewtonianViscousFlux>(s, n);
+  else dispatch_lim<false, ma::NoViscousFlux>(s, n);
+}
+
+ma_status download_aos(ma_solver* s, const double* dsrc, int ncomp, double* host) {
+  const int nc = s->m.nc;
+  std::vector<double> tmp((size_t)ncomp * nc);
+  MA_HIP(hipMemcpyAsync(tmp.data(), dsrc, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  MA_HIP(hipStreamSynchronize(s->stream));
+  for (int c = 0; c < nc; ++c)
+    for (int k = 0; k < ncomp; ++k) host[(size_t)c * ncomp + k] = tmp[(size_t)k * nc + c];
+  return MA_OK;
+}
+
+}  // namespace
+
+extern "C" const char* ma_last_error(void) { return ma::last_error().c_str(); }
+
+extern "C" ma_status ma_solver_create(const ma_mesh_data* md, const ma_options* opt, ma_solver** out) {
+  if (!md || !opt || !out) return ma::fail(MA_ERR_INVALID, "ma_solver_create: null argument");
+  *out = nullptr;
+  if (md->num_ghosts != 0)
+    return ma::fail(MA_ERR_UNSUPPORTED, "ghost cells (multi-rank meshes) are not supported on one GPU");
+  if (md->num_owned_cells <= 0 || !md->cell_coordinates || !md->cell_volumes)
+    return ma::fail(MA_ERR_INVALID, "ma_solver_create: empty or null cell arrays");
+  if (md->num_boundary_groups < 0 || (md->num_boundary_groups > 0 && !md->boundary_faces))
+    return ma::fail(MA_ERR_INVALID, "ma_solver_create: bad boundary group list");
+  if (opt->cfl <= 0.0 && !(opt->dt > 0.0)) return ma::fail(MA_ERR_INVALID, "dt must be positive");
+  if (opt->limiter != MA_LIMITER_VENKAT && opt->limiter != MA_LIMITER_VANALBADA)
+    return ma::fail(MA_ERR_INVALID, "unknown limiter");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return ma::fail(MA_ERR_DEVICE, "no HIP device available");
+  ma_solver* s = new (std::nothrow) ma_solver();
+  if (!s) return ma::fail(MA_ERR_NOMEM, "ma_solver_create: out of host memory");
+  s->opt = *opt;
+  // TimeSolverExplicitRK4.h:218-223
+  const double ref_inflow[5] = {0.5805, 503.96, 0.0, 0.0, 343750.0};
+  for (int k = 0; k < 5; ++k) s->inflow.U[k] = opt->use_inflow ? opt->inflow[k] : ref_inflow[k];
+  (void)hipGetDevice(&s->device);
+  ma_status st = MA_OK;
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return ma::fail(MA_ERR_DEVICE, "hipStreamCreate failed");
+  }
+  try {
+    st = build(s, md);
+  } catch (const std::bad_alloc&) {
+    st = ma::fail(MA_ERR_NOMEM, "ma_solver_create: out of host memory");
+  }
+  if (st) {
+    ma_solver_destroy(s);
+    return st;
+  }
+  *out = s;
+  return MA_OK;
+}
+
+extern "C" ma_status ma_solver_initialize(ma_solver* s) {
+  if (!s) return ma::fail(MA_ERR_INVALID, "null solver");
+  IcParams p{};
+  p.kind = s->opt.ic_kind;
+  p.problem_type = s->opt.problem_type;
+  p.midx = s->opt.lx / 2.0;
+  for (int k = 0; k < 5; ++k) {
+    p.ic[k] = s->opt.ic[k];
+    p.inflow[k] = s->inflow.U[k];
+  }
+  p.perturb = s->opt.perturb;
+  p.seed = s->opt.seed;
+  s->cur = 0;
+  hipLaunchKernelGGL(k_init, dim3(nblocks(s->m.nc)), dim3(kBlock), 0, s->stream, s->m, p, s->U[0]);
+  MA_HIP(hipGetLastError());
+  const size_t n5 = (size_t)5 * s->m.nc;
+  MA_HIP(hipMemsetAsync(s->R, 0, n5 * sizeof(double), s->stream));
+  MA_HIP(hipMemsetAsync(s->time_dev, 0, sizeof(double), s->stream));
+  MA_HIP(hipStreamSynchronize(s->stream));
+  s->time = 0.0;
+  s->last_dt = 0.0;
+  return MA_OK;
+}
+
+extern "C" ma_status ma_solver_set_state(ma_solver* s, const double* U) {
+  if (!s || !U) return ma::fail(MA_ERR_INVALID, "null argument");
+  const int nc = s->m.nc;
+  std::vector<double> tmp((size_t)5 * nc);
+  for (int c = 0; c < nc; ++c)
+    for (int k = 0; k < 5; ++k) tmp[(size_t)k * nc + c] = U[(size_t)c * 5 + k];
+  s->cur = 0;
+  MA_HIP(hipMemcpyAsync(s->U[0], tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  MA_HIP(hipMemsetAsync(s->R, 0, tmp.size() * sizeof(double), s->stream));
+  MA_HIP(hipStreamSynchronize(s->stream));
+  return MA_OK;
+}
+
+extern "C" ma_status ma_solver_step(ma_solver* s, int nsteps) {
+  if (!s) return ma::fail(MA_ERR_INVALID, "null solver");
+  if (nsteps < 0) return ma::fail(MA_ERR_INVALID, "negative step count");
+  dispatch(s, nsteps);
+  MA_HIP(hipGetLastError());
+  return MA_OK;
+}
+
+extern "C" ma_status ma_solver_synchronize(ma_solver* s) {
+  if (!s) return ma::fail(MA_ERR_INVALID, "null solver");
+  MA_HIP(hipStreamSynchronize(s->stream));
+  return MA_OK;
+}
+
+extern "C" ma_status ma_solver_get_state(ma_solver* s, double* U) {
+  if (!s || !U) return ma::fail(MA_ERR_INVALID, "null argument");
+  return download_aos(s, s->U[s->cur], 5, U);
+}
+
+extern "C" ma_status ma_solver_get_residual(ma_solver* s, double* R) {
+  if (!s || !R) return ma::fail(MA_ERR_INVALID, "null argument");
+  return download_aos(s, s->R, 5, R);
+}
+
+extern "C" ma_status ma_solver_get_gradients(ma_solver* s, double* grad, double* lim) {
+  if (!s) return ma::fail(MA_ERR_INVALID, "null solver");
+  ma_status st = MA_OK;
+  if (grad && (st = download_aos(s, s->grad, 15, grad))) return st;
+  if (lim && (st = download_aos(s, s->phi, 5, lim))) return st;
+  return MA_OK;
+}
+
+extern "C" ma_status ma_solver_get_time(ma_solver* s, double* time, double* last_dt) {
+  if (!s) return ma::fail(MA_ERR_INVALID, "null solver");
+  double t = s->time, d = s->last_dt;
+  if (s->opt.cfl > 0.0) {
+    MA_HIP(hipMemcpyAsync(&t, s->time_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    MA_HIP(hipMemcpyAsync(&d, s->dt_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    MA_HIP(hipStreamSynchronize(s->stream));
+  }
+  if (time) *time = t;
+  if (last_dt) *last_dt = d;
+  return MA_OK;
+}
+
+extern "C" ma_status ma_solver_check_finite(ma_solver* s, long long* nonfinite) {
+  if (!s || !nonfinite) return ma::fail(MA_ERR_INVALID, "null argument");
+  const int n = 5 * s->m.nc;
+  MA_HIP(hipMemsetAsync(s->counter, 0, sizeof(unsigned long long), s->stream));
+  hipLaunchKernelGGL(k_count_nonfinite, dim3(nblocks(n)), dim3(kBlock), 0, s->stream, n, s->U[s->cur],
+                     s->counter);
+  MA_HIP(hipGetLastError());
+  unsigned long long h = 0;
+  MA_HIP(hipMemcpyAsync(&h, s->counter, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+  MA_HIP(hipStreamSynchronize(s->stream));
+  *nonfinite = (long long)h;
+  return h ? ma::fail(MA_ERR_NONFINITE, "non-finite values in the solution") : MA_OK;
+}
+
+extern "C" ma_status ma_solver_stream(ma_solver* s, void** stream) {
+  if (!s || !stream) return ma::fail(MA_ERR_INVALID, "null argument");
+  *stream = (void*)s->stream;
+  return MA_OK;
+}
+
+extern "C" ma_status ma_solver_sizes(ma_solver* s, long long* counts) {
+  if (!s || !counts) return ma::fail(MA_ERR_INVALID, "null argument");
+  counts[0] = s->m.nc;
+  counts[1] = s->m.nif;
+  counts[2] = s->m.nf - s->m.nif;
+  counts[3] = (long long)s->bytes;
+  return MA_OK;
+}
+
+extern "C" int ma_solver_profile(ma_solver* s, int nsteps, const char** names, int* launches,
+                                 double* mean_ms, int max) {
+  if (!s || nsteps < 0) {
+    ma::fail(MA_ERR_INVALID, "ma_solver_profile: bad argument");
+    return -1;
+  }
+  (void)hipStreamSynchronize(s->stream);
+  s->timing.on = true;
+  s->timing.recs.clear();
+  dispatch(s, nsteps);
+  s->timing.on = false;
+  if (hipStreamSynchronize(s->stream) != hipSuccess) {
+    ma::fail(MA_ERR_DEVICE, "ma_solver_profile: stream error");
+    return -1;
+  }
+  std::vector<std::string> order;
+  std::map<std::string, std::pair<int, double>> agg;
+  for (auto& r : s->timing.recs) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, r.a, r.b);
+    if (!agg.count(r.name)) order.push_back(r.name);
+    agg[r.name].first += 1;
+    agg[r.name].second += ms;
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  s->timing.recs.clear();
+  static const char* kNames[] = {"k_prim", "k_grad", "k_grad_limiter", "k_flux", "k_dt_partial", "k_dt_final"};
+  int w = 0;
+  for (auto& n : order) {
+    if (w >= max) break;
+    const char* stable = nullptr;
+    for (const char* k : kNames)
+      if (n == k) stable = k;
+    names[w] = stable;
+    launches[w] = agg[n].first;
+    mean_ms[w] = agg[n].second / agg[n].first;
+    ++w;
+  }
+  return w;
+}
+
+extern "C" void ma_solver_destroy(ma_solver* s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (void* p : s->allocs) (void)hipFree(p);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}

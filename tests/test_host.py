@@ -1,0 +1,100 @@
+"""Host-side checks of the product that need no GPU: the C ABI library loads
+and exports every entry point the header declares; the closed-form mesh
+generator is bit-identical to the oracle's mesh (itself pinned to the
+reference's code); options reading, results formatting and the comparator."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import miniaero_amd as M
+from miniaero_amd import _lib
+from conftest import GOLDEN, ROOT
+from oracle import oracle as O
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    header = open(os.path.join(ROOT, "include", "miniaero.h")).read()
+    declared = set(re.findall(r"^(?:const char\*|ma_status|void|int)\s+(ma_[a-z_]+)\s*\(",
+                              header, re.M))
+    assert declared == set(_lib.EXPORTS)
+    for name in declared:
+        assert hasattr(L, name), name
+
+
+def test_no_gpu_is_a_clean_error_here():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    md = M.Parallel3DMesh(2, 2, 2, 1, 1, 1, 0).fillMeshData()
+    with pytest.raises(M.MiniAeroError) as e:
+        M.TimeSolverExplicitRK4(md, M.Options(dt=1e-6))
+    assert e.value.status == _lib.MA_ERR_DEVICE
+
+
+MESH_CASES = [
+    dict(problem_type=0, lx=0.3048, ly=1.0, lz=1.0, angle=0.0, nx=128, ny=4, nz=4),
+    dict(problem_type=2, lx=2.0, ly=2.0, lz=1.0, angle=30.0, nx=64, ny=32, nz=2),
+    dict(problem_type=1, lx=2.0, ly=0.002, lz=1.0, angle=0.0, nx=32, ny=64, nz=2),
+    dict(problem_type=1, lx=1.0, ly=0.05, lz=0.025, angle=0.0, nx=8, ny=6, nz=5, y_stretch=1.05,
+         bc_kind=(0, 2, 0, 0, 0, 0)),
+    dict(problem_type=0, lx=1.0, ly=1.0, lz=1.0, angle=0.0, nx=1, ny=3, nz=1),
+    dict(problem_type=2, lx=1.0, ly=1.0, lz=1.0, angle=10.0, nx=1, ny=1, nz=1),
+]
+
+
+def product_mesh(kw):
+    ext = {k: v for k, v in kw.items() if k in ("y_stretch", "bc_kind")}
+    return M.Parallel3DMesh(kw["nx"], kw["ny"], kw["nz"], kw["lx"], kw["ly"], kw["lz"],
+                            kw["problem_type"], kw["angle"], options=M.Options(**ext)).fillMeshData()
+
+
+@pytest.mark.parametrize("kw", MESH_CASES)
+def test_mesh_generator_bit_exact(kw):
+    om = O.OracleMesh(O.make_options(**kw))
+    md = product_mesh(kw)
+    assert md.num_owned_cells == om.ncells
+    assert np.array_equal(md.cell_coordinates, om.cell_xyz)
+    assert np.array_equal(md.cell_volumes, om.cell_vol)
+    pairs = [("xf", "coordinates"), ("a", "face_normal"), ("t", "face_tangent"),
+             ("b", "face_binormal"), ("conn", "face_cell_conn"), ("fidx", "cell_flux_index"),
+             ("perm", "permute_vector")]
+    for a, b in pairs:
+        assert np.array_equal(om.internal[a], getattr(md.internal_faces, b)), a
+    assert len(md.boundary_faces) == 6
+    for g, (name, F) in enumerate(md.boundary_faces):
+        assert _lib.BC_KINDS[name] == om.groups[g]["kind"]
+        for a, b in pairs:
+            assert np.array_equal(om.groups[g][a], getattr(F, b)), (g, a)
+
+
+def test_options_read(tmp_path):
+    o = M.Options().read_options_file(os.path.join(GOLDEN, "FlatPlate_Serial", "miniaero.inp"))
+    assert (o.problem_type, o.nx, o.ny, o.nz, o.ntimesteps) == (1, 16, 32, 2, 400)
+    assert (o.lx, o.ly, o.lz, o.angle, o.dt) == (2.0, 0.002, 1.0, 0.0, 3e-8)
+    assert (o.output_results, o.output_frequency, o.second_order_space, o.viscous) == (1, 100, 1, 1)
+    with pytest.raises(M.MiniAeroError):
+        M.Options().read_options_file(str(tmp_path / "missing.inp"))
+    short = tmp_path / "short.inp"
+    short.write_text("0\n1 1 1 0\n4 4\n")
+    with pytest.raises(M.MiniAeroError):
+        M.Options().read_options_file(str(short))
+
+
+def test_results_format_matches_gold_text():
+    gold = open(os.path.join(GOLDEN, "3D_Sod_Serial", "results.gold")).read()
+    rows = np.loadtxt(os.path.join(GOLDEN, "3D_Sod_Serial", "results.gold"))
+    text = M.results.format_rows(rows[:, :3], rows[:, 3:])
+    assert text == gold
+
+
+def test_numeric_text_diff_semantics():
+    a = "1\t2.5\t1e-07\t\n3\t4\t\n"
+    assert M.numeric_text_diff(a, a)[0]
+    assert M.numeric_text_diff(a, "1\t2.5001\t1e-07\t\n3\t4\t\n")[0]           # 4e-5 < 1e-3
+    assert not M.numeric_text_diff(a, "1\t2.6\t1e-07\t\n3\t4\t\n")[0]
+    assert M.numeric_text_diff(a, "1\t2.5\t9e-07\t\n3\t4\t\n")[0]              # below the floor
+    assert not M.numeric_text_diff(a, "1\t2.5\t9e-07\t\n3\t4\t\n", floor=1e-8)[0]
+    assert not M.numeric_text_diff(a, "1\t2.5\t1e-07\t\n")[0]                 # line count
+    assert not M.numeric_text_diff(a, "1\t2.5\t\n3\t4\t\n")[0]                # field count
