@@ -92,6 +92,8 @@ def main() -> int:
     ap.add_argument("--workload", default="flat_plate_c2")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-ceiling", action="store_true",
+                    help="only measure the plain copy kernel (1 GiB, read + write) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it "
@@ -99,7 +101,15 @@ def main() -> int:
         return 2
 
     import miniaero_amd as M
+    from miniaero_amd import _lib
     from miniaero_amd.configs import WORKLOADS
+
+    copy_bytes = 1 << 30
+    copy_ms = _lib.measure_copy(copy_bytes, 10)
+    copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
+    if args.copy_ceiling:
+        print(json.dumps({"copy_bytes": copy_bytes, "ms_per_copy": copy_ms, "GBps_read_plus_write": copy_gbs}))
+        return 0
 
     opts = WORKLOADS[args.workload]()
     t0 = time.perf_counter()
@@ -167,7 +177,8 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": dom, "algorithmic_bytes_per_launch": alg,
-                     "mean_launch_ms": round(mean_ms, 5)},
+                     "mean_launch_ms": round(mean_ms, 5), "copy_ceiling_GBps": round(copy_gbs, 1),
+                     "kernels": kernels},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -164,6 +164,11 @@ int ma_solver_profile(ma_solver* s, int nsteps, const char** names, int* launche
                       double* mean_ms, int max);
 /* Sizes: counts[0] cells, [1] internal faces, [2] boundary faces, [3] device bytes held. */
 ma_status ma_solver_sizes(ma_solver* s, long long* counts);
+/* Streaming-copy ceiling of the current GPU: copies `bytes` bytes `reps` times
+ * with a plain copy kernel and reports the mean time per copy (it reads and
+ * writes `bytes` each).  Used to calibrate HBM counters and roofline numbers;
+ * no reference counterpart. */
+ma_status ma_measure_copy(long long bytes, int reps, double* ms_per_copy);
 
 #ifdef __cplusplus
 }

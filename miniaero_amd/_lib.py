@@ -31,6 +31,7 @@ EXPORTS = (
     "ma_solver_synchronize", "ma_solver_get_state", "ma_solver_get_residual",
     "ma_solver_get_gradients", "ma_solver_get_time", "ma_solver_check_finite",
     "ma_solver_stream", "ma_solver_destroy", "ma_solver_profile", "ma_solver_sizes",
+    "ma_measure_copy",
 )
 
 
@@ -132,8 +133,17 @@ def load() -> ctypes.CDLL:
     L.ma_solver_profile.restype = ctypes.c_int
     L.ma_solver_profile.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _I, _D, ctypes.c_int]
     L.ma_solver_sizes.argtypes = [P, ctypes.POINTER(ctypes.c_longlong)]
+    L.ma_measure_copy.argtypes = [ctypes.c_longlong, ctypes.c_int, _D]
     _lib = L
     return L
+
+
+def measure_copy(nbytes: int, reps: int = 20) -> float:
+    """Mean milliseconds of one device-to-device copy of nbytes by a plain
+    copy kernel (reads and writes nbytes)."""
+    ms = ctypes.c_double(0.0)
+    check(load().ma_measure_copy(int(nbytes), int(reps), ctypes.byref(ms)))
+    return ms.value
 
 
 def check(status: int) -> None:

@@ -49,12 +49,45 @@ MA_HD double enthalpy(const double* V) { return 1004.0 * V[4]; }        // GasMo
 MA_HD double smin(double a, double b) { return (b < a) ? b : a; }
 MA_HD double smax(double a, double b) { return (a < b) ? b : a; }
 
+// Geometry-only part of the Roe flux: the area vector, its norm and the unit
+// normal, tangent and binormal (Roe_Flux.h:100-118).  It depends on the face
+// alone, so the solver computes it once per face at setup, with the same
+// operations, instead of once per flux evaluation.
+struct FaceFrame {
+  double n[3];   // area-weighted normal
+  double nn;     // |n|
+  double nu[3], tu[3], bu[3];
+};
+
+MA_HD FaceFrame make_frame(const double* fn, const double* ft, const double* fb) {
+  FaceFrame f;
+  const double nn = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+  const double tn = sqrt(ft[0] * ft[0] + ft[1] * ft[1] + ft[2] * ft[2]);
+  const double bn = sqrt(fb[0] * fb[0] + fb[1] * fb[1] + fb[2] * fb[2]);
+  for (int d = 0; d < 3; ++d) {
+    f.n[d] = fn[d];
+    f.nu[d] = fn[d] / nn;
+    f.tu[d] = ft[d] / tn;
+    f.bu[d] = fb[d] / bn;
+  }
+  f.nn = nn;
+  return f;
+}
+
 struct RoeFlux {
+  // Roe_Flux.h:50-54 signature: builds the face frame, then the flux.
+  MA_HD void compute_flux(const double* pl, const double* pr, double* flux, const double* fn,
+                          const double* ft, const double* fb) const {
+    const FaceFrame fr = make_frame(fn, ft, fb);
+    flux_frame(pl, pr, flux, fr);
+  }
+
   // Roe flux with the eigenvalue fix of Roe_Flux.h:157-186.  The 5x5 left and
   // right eigenvector products are unrolled, each row summed in column order
   // from zero as MathTools::MatVec5 does (MathToolsDevice.h:62-70).
-  MA_HD void compute_flux(const double* pl, const double* pr, double* flux, const double* fn,
-                          const double* ft, const double* fb) const {
+  MA_HD void flux_frame(const double* pl, const double* pr, double* flux,
+                        const FaceFrame& fr) const {
+    const double* fn = fr.n;
     const double efix = 0.1;
     const double gm1 = 0.4;
     const double rl = pl[0], ul = pl[1], vl = pl[2], wl = pl[3];
@@ -72,12 +105,10 @@ struct RoeFlux {
     double f3 = 0.5 * (mf_l * wl + mf_r * wr + fn[2] * psum);
     double f4 = 0.5 * (mf_l * ht_l + mf_r * ht_r);
 
-    const double nn = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
-    const double tn = sqrt(ft[0] * ft[0] + ft[1] * ft[1] + ft[2] * ft[2]);
-    const double bn = sqrt(fb[0] * fb[0] + fb[1] * fb[1] + fb[2] * fb[2]);
-    const double n0 = fn[0] / nn, n1 = fn[1] / nn, n2 = fn[2] / nn;
-    const double t0 = ft[0] / tn, t1 = ft[1] / tn, t2 = ft[2] / tn;
-    const double b0 = fb[0] / bn, b1 = fb[1] / bn, b2 = fb[2] / bn;
+    const double nn = fr.nn;
+    const double n0 = fr.nu[0], n1 = fr.nu[1], n2 = fr.nu[2];
+    const double t0 = fr.tu[0], t1 = fr.tu[1], t2 = fr.tu[2];
+    const double b0 = fr.bu[0], b1 = fr.bu[1], b2 = fr.bu[2];
 
     const double srl = sqrt(rl);
     const double denom = 1.0 / (srl + sqrt(rr));

@@ -11,27 +11,31 @@
 //   k_prim   U_s = U_n + alpha_k R;  V = primitives(U_s)           (per cell)
 //   k_grad   Green-Gauss gradient, stencil min/max and the limiter  (per cell)
 //            of every cell, from V of the cell and its face neighbours
-//   k_flux   every face flux of the cell (Roe [+ reconstruction]    (per cell)
-//            [+ viscous]), boundary fluxes, residual R = dt/V sum,
-//            RK accumulate U_acc += beta_k R
+//   k_bc     boundary-face contributions (Roe of the ghost state    (per boundary face)
+//            [+ no-slip viscous flux])
+//   k_flux   per tile of 256 cells: every internal face touching    (per cell tile)
+//            the tile evaluated once into LDS (Roe [+ reconstruction]
+//            [+ viscous]), then per cell: sum of face and boundary
+//            contributions, residual R = dt/V sum, U_acc += beta_k R
 //
 // Determinism and parity.  The reference scatters face results into cells
 // with atomics; with one thread the additions land in a fixed order: internal
 // faces in the permuted order (stable by left cell), then boundary faces.  At
 // setup every cell gets its face list in exactly that order (slot table), and
-// each kernel sums its cell's contributions in slot order.  An internal face
-// flux is evaluated by both of its cells -- bitwise the same value, since it is
-// a pure function of the same inputs -- so no cell ever waits on another and
-// the result does not depend on scheduling.  With -ffp-contract=off the FP64
+// each kernel sums its cell's contributions in slot order.  A face between two
+// tiles is evaluated by both tiles -- bitwise the same value, since it is a
+// pure function of the same inputs -- so no workgroup ever waits on another
+// and the result does not depend on scheduling.  With -ffp-contract=off the FP64
 // results equal those of a one-thread CPU run of the reference.
 //
 // Layout in HBM (structure of arrays, one plane per component, so a
 // wavefront's 64 lanes read 64 consecutive doubles):
 //   cells   vol[nc], xc[3][nc], U[2][5][nc] (U_n / U_acc ping-pong), R[5][nc],
 //           V[5][nc], grad[15][nc], phi[5][nc], slot[6][nc] (int32)
-//   faces   n[3][nf], t[3][nf], b[3][nf], xf[3][nf], left[nf], right[nf],
-//           kind[nf] (int8), wall[nf]; internal faces stored in processing
-//           (left-cell) order, then the boundary groups
+//   faces   n[3][nf], frame[10][nf] (|n|, unit n, t, b: geometry-only part
+//           of the Roe flux, computed once at setup), xf[3][nf], left[nf],
+//           right[nf], kind[nf] (int8), wall[nf]; internal faces stored in
+//           processing (left-cell) order, then the boundary groups
 // Entry point declarations and the reference interfaces they replace are in
 // include/miniaero.h.
 #include <hip/hip_runtime.h>
@@ -53,6 +57,7 @@ namespace {
 
 constexpr int kMaxSlots = 6;  // hex8 cells (ElementTopoHexa8); checked at setup
 constexpr int kBlock = 256;
+constexpr int kTile = 256;    // cells per workgroup of the flux kernel
 
 struct DevMesh {
   int nc;    // owned cells
@@ -61,9 +66,8 @@ struct DevMesh {
   const double* __restrict__ vol;
   const double* __restrict__ xc;   // 3 planes of nc
   const int* __restrict__ slot;    // kMaxSlots planes of nc; entry = face*2 + side, -1 empty
-  const double* __restrict__ fn;   // 3 planes of nf
-  const double* __restrict__ ft;
-  const double* __restrict__ fb;
+  const double* __restrict__ fn;   // 3 planes of nf: area vector
+  const double* __restrict__ ffr;  // 10 planes of nf: |n|, unit n, unit t, unit b
   const double* __restrict__ fx;
   const int* __restrict__ fl;
   const int* __restrict__ fr;
@@ -75,6 +79,16 @@ struct Inflow {
   double U[5];
 };
 
+// Tiles of the flux kernel: tile b holds cells [b*kTile, (b+1)*kTile) and the
+// internal faces faces[ptr[b] .. ptr[b+1]) that touch them (ascending).
+struct TileMap {
+  int ntiles;
+  int max_faces;                  // largest ptr[b+1] - ptr[b]
+  const int* __restrict__ ptr;    // ntiles + 1
+  const int* __restrict__ faces;  // device face indices
+  const int* __restrict__ slot;   // kMaxSlots planes of nc, see k_flux
+};
+
 __device__ __forceinline__ void load5(const double* __restrict__ a, int n, int i, double* v) {
 #pragma unroll
   for (int k = 0; k < 5; ++k) v[k] = a[(size_t)k * n + i];
@@ -83,6 +97,21 @@ __device__ __forceinline__ void load5(const double* __restrict__ a, int n, int i
 __device__ __forceinline__ void load3(const double* __restrict__ a, int n, int i, double* v) {
 #pragma unroll
   for (int k = 0; k < 3; ++k) v[k] = a[(size_t)k * n + i];
+}
+
+__device__ __forceinline__ ma::FaceFrame load_frame(const DevMesh& m, int f) {
+  ma::FaceFrame fr;
+  load3(m.fn, m.nf, f, fr.n);
+  const double* p = m.ffr;
+  const size_t nf = m.nf;
+  fr.nn = p[f];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    fr.nu[d] = p[(1 + d) * nf + f];
+    fr.tu[d] = p[(4 + d) * nf + f];
+    fr.bu[d] = p[(7 + d) * nf + f];
+  }
+  return fr;
 }
 
 // ---------------------------------------------------------------- kernels --
@@ -307,134 +336,179 @@ __device__ __forceinline__ void reconstruct(const DevMesh& m, const double* __re
   }
 }
 
-// All face fluxes of one cell, residual and RK accumulate:
-// compute_face_flux (Flux.h:52-182), the four boundary functors
-// (Extrapolate_BC.h:45-102, Tangent_BC.h:47-123, NoSlip_BC.h:47-157,
-// Inflow_BC.h:45-111) in the reference's kind order, apply_cell_flux
-// (Flux.h:188-229) and update<Device> with beta_k (TimeSolverExplicitRK4.h:483-485).
+// Flux of internal face f: compute_face_flux's body (Flux.h:89-160) without
+// the scatter -- primitives of both cells, second-order extrapolation with
+// the cell limiters, Roe, minus the viscous flux from face-averaged
+// gradients.  The result is a pure function of (f, V, grad, phi), so every
+// workgroup that evaluates face f gets the same bits.
 template <bool kSecondOrder, class Visc>
-__global__ __launch_bounds__(kBlock) void k_flux(DevMesh m, const double* __restrict__ V,
-                                                 const double* __restrict__ grad,
-                                                 const double* __restrict__ phi,
-                                                 const double* Uacc_in,  // may alias Uacc_out
-                                                 double* Uacc_out,
-                                                 double* __restrict__ R,
-                                                 const double* __restrict__ dt_dev, double beta,
-                                                 Visc visc, Inflow inflow) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= m.nc) return;
-  const int nc = m.nc, nf = m.nf;
+__device__ __forceinline__ void interior_flux(const DevMesh& m, const double* __restrict__ V,
+                                              const double* __restrict__ grad,
+                                              const double* __restrict__ phi, int f,
+                                              const Visc& visc, double* F) {
+  const int nc = m.nc;
+  const int L = m.fl[f], Rr = m.fr[f];
+  double vl[5], vr[5];
+  load5(V, nc, L, vl);
+  load5(V, nc, Rr, vr);
+  const ma::FaceFrame fr = load_frame(m, f);
+  if (kSecondOrder) {
+    double xf[3];
+    load3(m.fx, m.nf, f, xf);
+    reconstruct(m, grad, phi, L, xf, vl);
+    reconstruct(m, grad, phi, Rr, xf, vr);
+  }
   const ma::RoeFlux roe;
-  double vc[5];
-  load5(V, nc, c, vc);
+  roe.flux_frame(vl, vr, F, fr);
+  if (Visc::isViscous) {
+    double vf[5], gf[5][3], fv[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      vf[k] = 0.5 * (vl[k] + vr[k]);
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        gf[k][d] = 0.5 * (grad[(size_t)(k * 3 + d) * nc + L] + grad[(size_t)(k * 3 + d) * nc + Rr]);
+    }
+    visc.compute_flux(gf, vf, fr.n, fv);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) F[k] -= fv[k];
+  }
+}
+
+// Boundary faces, one thread each: the contribution each adds to its cell's
+// flux sum -- -F for Extrapolate_BC.h:69-101, Tangent_BC.h:71-122 and
+// Inflow_BC.h:72-110, -F_inviscid + F_viscous for NoSlip_BC.h:76-156 (whose
+// newtonian viscous part is applied even in inviscid runs,
+// TimeSolverExplicitRK4.h:457-460).  Written to bc[5][nbf].
+__global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const double* __restrict__ V,
+                                               double prandtl, Inflow inflow,
+                                               double* __restrict__ bc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nbf = m.nf - m.nif;
+  if (i >= nbf) return;
+  const int f = m.nif + i;
+  const int c = m.fl[f];
+  const int kind = m.fkind[f];
+  double vc[5], vr[5], F[5];
+  load5(V, m.nc, c, vc);
+  const ma::FaceFrame fr = load_frame(m, f);
+  const ma::RoeFlux roe;
+  double out[5];
+  if (kind == MA_BC_EXTRAPOLATE) {
+    roe.flux_frame(vc, vc, F, fr);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) out[k] = -F[k];
+  } else if (kind == MA_BC_INFLOW) {
+    ma::primitives(inflow.U, vr);
+    roe.flux_frame(vc, vr, F, fr);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) out[k] = -F[k];
+  } else {
+    // mirror state; area_norm (Tangent_BC.h:87-91) equals |n| of the frame bit for bit
+    const double* n = fr.n;
+    const double an = fr.nn;
+    double ub = 0.0;
+    ub += vc[1] * n[0] / an;
+    ub += vc[2] * n[1] / an;
+    ub += vc[3] * n[2] / an;
+    vr[0] = vc[0];
+    vr[1] = vc[1] - 2 * ub * n[0] / an;
+    vr[2] = vc[2] - 2 * ub * n[1] / an;
+    vr[3] = vc[3] - 2 * ub * n[2] / an;
+    vr[4] = vc[4];
+    roe.flux_frame(vc, vr, F, fr);
+    if (kind == MA_BC_TANGENT) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) out[k] = -F[k];
+    } else {
+      // no-slip wall: viscous flux from the one-sided gradient to the wall
+      // state (rho, 0, 0, 0, T) over the centroid-to-face distance
+      const double vf[5] = {vc[0], 0.0, 0.0, 0.0, vc[4]};
+      const double inv = m.fwall[f];
+      double gf[5][3], fv[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gf[k][d] = (vf[k] - vc[k]) * fr.nu[d] * inv;
+      const ma::NewtonianViscousFlux wall{prandtl};
+      wall.compute_flux(gf, vf, n, fv);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) out[k] = -F[k] + fv[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) bc[(size_t)k * nbf + i] = out[k];
+}
+
+// Residual and RK accumulate of one tile of kTile consecutive cells:
+// compute_face_flux's atomic scatter (Flux.h:162-170), apply_cell_flux
+// (Flux.h:188-229) and update<Device> with beta_k (TimeSolverExplicitRK4.h:483-485).
+// Phase 1: the workgroup evaluates every internal face touching its cells
+// once into LDS (faces between two tiles are evaluated by both).  Phase 2:
+// every cell adds its face fluxes from LDS and its boundary contributions in
+// the reference's one-thread order -- internal faces in processing order,
+// then boundary faces by kind (Extrapolate, Tangent, NoSlip, Inflow) and group.
+// Slot entries: (index << 3) | code, code 0 = left side, 1 = right side of an
+// internal face (index = LDS slot), 4 + kind - 1 = boundary face of that kind
+// (index = boundary-face number).
+template <bool kSecondOrder, class Visc>
+__global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const double* __restrict__ V,
+                                                const double* __restrict__ grad,
+                                                const double* __restrict__ phi,
+                                                const double* __restrict__ bc,
+                                                const double* Uacc_in,  // may alias Uacc_out
+                                                double* Uacc_out, double* __restrict__ R,
+                                                const double* __restrict__ dt_dev, double beta,
+                                                Visc visc) {
+  extern __shared__ double lds[];  // 5 planes of t.max_faces
+  const int tile = blockIdx.x;
+  const int f0 = t.ptr[tile];
+  const int nfl = t.ptr[tile + 1] - f0;
+  const int mf = t.max_faces;
+  for (int i = threadIdx.x; i < nfl; i += kTile) {
+    double F[5];
+    interior_flux<kSecondOrder, Visc>(m, V, grad, phi, t.faces[f0 + i], visc, F);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) lds[k * mf + i] = F[k];
+  }
+  __syncthreads();
+  const int c = tile * kTile + threadIdx.x;
+  if (c >= m.nc) return;
+  const int nc = m.nc;
+  const int nbf = m.nf - m.nif;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   int ent[kMaxSlots];
 #pragma unroll
-  for (int s = 0; s < kMaxSlots; ++s) ent[s] = m.slot[(size_t)s * nc + c];
-
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-
+  for (int s = 0; s < kMaxSlots; ++s) ent[s] = t.slot[(size_t)s * nc + c];
+  bool any_bc = false;
 #pragma unroll
   for (int s = 0; s < kMaxSlots; ++s) {
     const int e = ent[s];
-    if (e < 0 || (e >> 1) >= m.nif) break;
-    const int f = e >> 1, side = e & 1;
-    const int L = m.fl[f], Rr = m.fr[f];
-    double vl[5], vr[5];
-    load5(V, nc, L, vl);
-    load5(V, nc, Rr, vr);
-    double n[3], t[3], b[3];
-    load3(m.fn, nf, f, n);
-    load3(m.ft, nf, f, t);
-    load3(m.fb, nf, f, b);
-    if (kSecondOrder) {
-      double xf[3];
-      load3(m.fx, nf, f, xf);
-      reconstruct(m, grad, phi, L, xf, vl);
-      reconstruct(m, grad, phi, Rr, xf, vr);
-    }
-    double F[5];
-    roe.compute_flux(vl, vr, F, n, t, b);
-    if (Visc::isViscous) {
-      double vf[5], gf[5][3], fv[5];
+    if (e < 0) break;
+    const int code = e & 7, j = e >> 3;
+    if (code < 2) {
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
-        vf[k] = 0.5 * (vl[k] + vr[k]);
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-          gf[k][d] = 0.5 * (grad[(size_t)(k * 3 + d) * nc + L] + grad[(size_t)(k * 3 + d) * nc + Rr]);
+        const double x = lds[k * mf + j];
+        acc[k] += code ? x : -x;
       }
-      visc.compute_flux(gf, vf, n, fv);
-#pragma unroll
-      for (int k = 0; k < 5; ++k) F[k] -= fv[k];
+    } else {
+      any_bc = true;
     }
-#pragma unroll
-    for (int k = 0; k < 5; ++k) acc[k] += side ? F[k] : -F[k];
   }
-
-  // Boundary faces: Extrapolate, Tangent, NoSlip, Inflow, each in group order.
-  bool any_bc = false;
-#pragma unroll
-  for (int s = 0; s < kMaxSlots; ++s) any_bc |= ent[s] >= 0 && (ent[s] >> 1) >= m.nif;
   if (any_bc) {
-    for (int kind = MA_BC_EXTRAPOLATE; kind <= MA_BC_INFLOW; ++kind) {
+#pragma unroll 1
+    for (int code = 4; code < 8; ++code)
+#pragma unroll 1
       for (int s = 0; s < kMaxSlots; ++s) {
         const int e = ent[s];
         if (e < 0) break;
-        const int f = e >> 1;
-        if (f < m.nif || m.fkind[f] != kind) continue;
-        double n[3], t[3], b[3], vr[5], F[5];
-        load3(m.fn, nf, f, n);
-        load3(m.ft, nf, f, t);
-        load3(m.fb, nf, f, b);
-        if (kind == MA_BC_EXTRAPOLATE) {
-          roe.compute_flux(vc, vc, F, n, t, b);
+        if ((e & 7) != code) continue;
+        const int j = e >> 3;
 #pragma unroll
-          for (int k = 0; k < 5; ++k) acc[k] += -F[k];
-        } else if (kind == MA_BC_INFLOW) {
-          ma::primitives(inflow.U, vr);
-          roe.compute_flux(vc, vr, F, n, t, b);
-#pragma unroll
-          for (int k = 0; k < 5; ++k) acc[k] += -F[k];
-        } else {
-          double an = 0;
-#pragma unroll
-          for (int d = 0; d < 3; ++d) an += n[d] * n[d];
-          an = sqrt(an);
-          double ub = 0.0;
-          ub += vc[1] * n[0] / an;
-          ub += vc[2] * n[1] / an;
-          ub += vc[3] * n[2] / an;
-          vr[0] = vc[0];
-          vr[1] = vc[1] - 2 * ub * n[0] / an;
-          vr[2] = vc[2] - 2 * ub * n[1] / an;
-          vr[3] = vc[3] - 2 * ub * n[2] / an;
-          vr[4] = vc[4];
-          roe.compute_flux(vc, vr, F, n, t, b);
-          if (kind == MA_BC_TANGENT) {
-#pragma unroll
-            for (int k = 0; k < 5; ++k) acc[k] += -F[k];
-          } else {
-            // no-slip wall: newtonian viscous flux from the one-sided wall
-            // gradient, applied even in inviscid runs (TimeSolverExplicitRK4.h:457-460)
-            const double vf[5] = {vc[0], 0.0, 0.0, 0.0, vc[4]};
-            const double inv = m.fwall[f];
-            double gf[5][3], fv[5], un[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) un[d] = n[d] / an;
-#pragma unroll
-            for (int k = 0; k < 5; ++k)
-#pragma unroll
-              for (int d = 0; d < 3; ++d) gf[k][d] = (vf[k] - vc[k]) * un[d] * inv;
-            ma::NewtonianViscousFlux wall{visc.prandtl};
-            wall.compute_flux(gf, vf, n, fv);
-#pragma unroll
-            for (int k = 0; k < 5; ++k) acc[k] += -F[k] + fv[k];
-          }
-        }
+        for (int k = 0; k < 5; ++k) acc[k] += bc[(size_t)k * nbf + j];
       }
-    }
   }
-
   const double dt = *dt_dev;
   const double vol = m.vol[c];
 #pragma unroll
@@ -509,6 +583,14 @@ __global__ __launch_bounds__(kBlock) void k_count_nonfinite(int n, const double*
   if ((threadIdx.x & 63) == 0 && mask) atomicAdd(out, (unsigned long long)__popcll(mask));
 }
 
+// Plain streaming copy, the bandwidth ceiling the kernels above are compared
+// with (16-byte loads and stores, grid-stride).
+__global__ __launch_bounds__(kBlock) void k_copy(long long n2, const double2* __restrict__ a,
+                                                 double2* __restrict__ b) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) b[i] = a[i];
+}
+
 // ------------------------------------------------------------- host side --
 
 #define MA_HIP(call)                                                                      \
@@ -534,6 +616,8 @@ struct ma_solver {
   int device = 0;
   hipStream_t stream = nullptr;
   DevMesh m{};
+  TileMap t{};
+  double* bc = nullptr;  // boundary-face contributions, 5 planes of nbf
   std::vector<void*> allocs;
   size_t bytes = 0;
   double *U[2] = {nullptr, nullptr}, *R = nullptr, *V = nullptr, *grad = nullptr, *phi = nullptr;
@@ -644,18 +728,22 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
   // Device face order: internal faces in processing order, then every
   // boundary group in order.
   const std::vector<int> perm = processing_order(I, nc);
-  std::vector<double> fn((size_t)3 * nf), ft((size_t)3 * nf), fb((size_t)3 * nf), fx((size_t)3 * nf);
+  std::vector<double> fn((size_t)3 * nf), ffr((size_t)10 * nf), fx((size_t)3 * nf);
   std::vector<int> fl(nf), fr(nf);
   std::vector<signed char> fkind(nf, 0);
   std::vector<double> fwall(nf, 0.0);
   std::vector<int> slot((size_t)kMaxSlots * nc, -1);
   std::vector<unsigned char> used(nc, 0);
   auto put = [&](int f, const ma_faces& F, int i) {
+    const ma::FaceFrame fr = ma::make_frame(&F.face_normal[3 * i], &F.face_tangent[3 * i],
+                                            &F.face_binormal[3 * i]);
+    ffr[f] = fr.nn;
     for (int d = 0; d < 3; ++d) {
       fn[(size_t)d * nf + f] = F.face_normal[3 * i + d];
-      ft[(size_t)d * nf + f] = F.face_tangent[3 * i + d];
-      fb[(size_t)d * nf + f] = F.face_binormal[3 * i + d];
       fx[(size_t)d * nf + f] = F.coordinates[3 * i + d];
+      ffr[(size_t)(1 + d) * nf + f] = fr.nu[d];
+      ffr[(size_t)(4 + d) * nf + f] = fr.tu[d];
+      ffr[(size_t)(7 + d) * nf + f] = fr.bu[d];
     }
   };
   auto add_slot = [&](int c, int entry) -> bool {
@@ -690,6 +778,46 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
         return ma::fail(MA_ERR_UNSUPPORTED, "a cell has more than 6 faces (hex8 meshes only)");
     }
   }
+  // Tiles of the flux kernel.  Every internal face is listed in the tile of
+  // its left cell and, when different, in the tile of its right cell; faces
+  // are appended in ascending device order, so each list is sorted.
+  const int ntiles = (nc + kTile - 1) / kTile;
+  std::vector<int> tptr((size_t)ntiles + 1, 0);
+  for (int g = 0; g < nif; ++g) {
+    const int tl = fl[g] / kTile, tr = fr[g] / kTile;
+    tptr[tl + 1]++;
+    if (tr != tl) tptr[tr + 1]++;
+  }
+  for (int b = 0; b < ntiles; ++b) tptr[b + 1] += tptr[b];
+  std::vector<int> tfaces((size_t)std::max(1, tptr[ntiles])), locl(nif), locr(nif);
+  {
+    std::vector<int> fill(tptr.begin(), tptr.end() - 1);
+    for (int g = 0; g < nif; ++g) {
+      const int tl = fl[g] / kTile, tr = fr[g] / kTile;
+      locl[g] = fill[tl] - tptr[tl];
+      tfaces[fill[tl]++] = g;
+      if (tr != tl) {
+        locr[g] = fill[tr] - tptr[tr];
+        tfaces[fill[tr]++] = g;
+      } else {
+        locr[g] = locl[g];
+      }
+    }
+  }
+  int max_faces = 1;
+  for (int b = 0; b < ntiles; ++b) max_faces = std::max(max_faces, tptr[b + 1] - tptr[b]);
+  if ((size_t)max_faces * 5 * sizeof(double) > 65536)
+    return ma::fail(MA_ERR_UNSUPPORTED, "too many faces per cell tile for the flux kernel's LDS");
+  std::vector<int> tslot((size_t)kMaxSlots * nc, -1);
+  for (int c = 0; c < nc; ++c)
+    for (int q = 0; q < kMaxSlots; ++q) {
+      const int e = slot[(size_t)q * nc + c];
+      if (e < 0) break;
+      const int g = e >> 1, side = e & 1;
+      tslot[(size_t)q * nc + c] =
+          g < nif ? (((side ? locr[g] : locl[g]) << 3) | side) : (((g - nif) << 3) | (3 + fkind[g]));
+    }
+
   std::vector<double> vol(md->cell_volumes, md->cell_volumes + nc);
   std::vector<double> xc((size_t)3 * nc);
   for (int c = 0; c < nc; ++c)
@@ -699,11 +827,11 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
   m.nc = nc;
   m.nif = nif;
   m.nf = nf;
-  double *dvol, *dxc, *dfn, *dft, *dfb, *dfx, *dfw;
+  double *dvol, *dxc, *dfn, *dffr, *dfx, *dfw;
   int *dslot, *dfl, *dfr;
   signed char* dkind;
   if ((st = upload(s, &dvol, vol)) || (st = upload(s, &dxc, xc)) || (st = upload(s, &dslot, slot)) ||
-      (st = upload(s, &dfn, fn)) || (st = upload(s, &dft, ft)) || (st = upload(s, &dfb, fb)) ||
+      (st = upload(s, &dfn, fn)) || (st = upload(s, &dffr, ffr)) ||
       (st = upload(s, &dfx, fx)) || (st = upload(s, &dfl, fl)) || (st = upload(s, &dfr, fr)) ||
       (st = upload(s, &dkind, fkind)) || (st = upload(s, &dfw, fwall)))
     return st;
@@ -711,13 +839,22 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
   m.xc = dxc;
   m.slot = dslot;
   m.fn = dfn;
-  m.ft = dft;
-  m.fb = dfb;
+  m.ffr = dffr;
   m.fx = dfx;
   m.fl = dfl;
   m.fr = dfr;
   m.fkind = dkind;
   m.fwall = dfw;
+
+  int *dtptr, *dtfaces, *dtslot;
+  if ((st = upload(s, &dtptr, tptr)) || (st = upload(s, &dtfaces, tfaces)) || (st = upload(s, &dtslot, tslot)) ||
+      (st = s->alloc(&s->bc, (size_t)5 * std::max<long long>(nbf, 1))))
+    return st;
+  s->t.ntiles = ntiles;
+  s->t.max_faces = max_faces;
+  s->t.ptr = dtptr;
+  s->t.faces = dtfaces;
+  s->t.slot = dtslot;
 
   const size_t n5 = (size_t)5 * nc;
   if ((st = s->alloc(&s->U[0], n5)) || (st = s->alloc(&s->U[1], n5)) || (st = s->alloc(&s->R, n5)) ||
@@ -753,9 +890,17 @@ void run_stage(ma_solver* s, int irk) {
   Visc visc{s->opt.prandtl > 0.0 ? s->opt.prandtl : 0.71};
   // Stage 0 accumulates from sol_n (sol_np1 == sol_n at the start of a step).
   const double* acc_in = irk == 0 ? Un : Uacc;
+  const int nbf = m.nf - m.nif;
+  if (nbf > 0) {
+    s->launch("k_bc", [&] {
+      hipLaunchKernelGGL(k_bc, dim3(nblocks(nbf)), dim3(kBlock), 0, s->stream, m, s->V, visc.prandtl,
+                         s->inflow, s->bc);
+    });
+  }
+  const size_t lds = (size_t)s->t.max_faces * 5 * sizeof(double);
   s->launch("k_flux", [&] {
-    hipLaunchKernelGGL((k_flux<SO, Visc>), dim3(nb), dim3(kBlock), 0, s->stream, m, s->V, s->grad,
-                       s->phi, acc_in, Uacc, s->R, s->dt_dev, beta[irk], visc, s->inflow);
+    hipLaunchKernelGGL((k_flux<SO, Visc>), dim3(s->t.ntiles), dim3(kTile), lds, s->stream, m, s->t,
+                       s->V, s->grad, s->phi, s->bc, acc_in, Uacc, s->R, s->dt_dev, beta[irk], visc);
   });
 }
 
@@ -988,7 +1133,8 @@ extern "C" int ma_solver_profile(ma_solver* s, int nsteps, const char** names, i
     (void)hipEventDestroy(r.b);
   }
   s->timing.recs.clear();
-  static const char* kNames[] = {"k_prim", "k_grad", "k_grad_limiter", "k_flux", "k_dt_partial", "k_dt_final"};
+  static const char* kNames[] = {"k_prim", "k_grad", "k_grad_limiter", "k_bc", "k_flux",
+                                 "k_dt_partial", "k_dt_final"};
   int w = 0;
   for (auto& n : order) {
     if (w >= max) break;
@@ -1001,6 +1147,38 @@ extern "C" int ma_solver_profile(ma_solver* s, int nsteps, const char** names, i
     ++w;
   }
   return w;
+}
+
+extern "C" ma_status ma_measure_copy(long long bytes, int reps, double* ms_per_copy) {
+  if (bytes < 16 || reps < 1 || !ms_per_copy) return ma::fail(MA_ERR_INVALID, "ma_measure_copy: bad argument");
+  const long long n2 = bytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ma_status rc = MA_OK;
+  float ms = 0.f;
+  if (hipMalloc((void**)&a, n2 * 16) != hipSuccess || hipMalloc((void**)&b, n2 * 16) != hipSuccess) {
+    rc = ma::fail(MA_ERR_NOMEM, "ma_measure_copy: hipMalloc failed");
+  } else if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+             hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = ma::fail(MA_ERR_DEVICE, "ma_measure_copy: stream/event creation failed");
+  } else {
+    const int grid = 256 * 16;  // 16 blocks per CU
+    (void)hipMemsetAsync(a, 0, n2 * 16, st);
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, st, n2, a, b);  // warm-up
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, st, n2, a, b);
+    (void)hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = ma::fail(MA_ERR_DEVICE, "ma_measure_copy: kernel failed");
+    *ms_per_copy = ms / reps;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
 }
 
 extern "C" void ma_solver_destroy(ma_solver* s) {
