@@ -98,3 +98,11 @@ def test_numeric_text_diff_semantics():
     assert not M.numeric_text_diff(a, "1\t2.5\t9e-07\t\n3\t4\t\n", floor=1e-8)[0]
     assert not M.numeric_text_diff(a, "1\t2.5\t1e-07\t\n")[0]                 # line count
     assert not M.numeric_text_diff(a, "1\t2.5\t\n3\t4\t\n")[0]                # field count
+
+
+def test_cpp_driver_reports_missing_input(tmp_path):
+    import subprocess
+    exe = os.path.join(ROOT, "miniaero_amd", "bin", "miniaero")
+    assert os.path.exists(exe), "bin/miniaero not built"
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "miniaero.inp not found" in r.stderr
