@@ -6,11 +6,10 @@
 
 namespace ma {
 
-// Thread-local message behind ma_last_error().
-inline std::string& last_error() {
-  static thread_local std::string msg;
-  return msg;
-}
+// Thread-local message behind ma_last_error().  Defined once, in mesh.cpp:
+// the library mixes g++- and hipcc-compiled objects, and an inline function's
+// static would not reliably be shared between them.
+std::string& last_error();
 
 inline ma_status fail(ma_status code, const std::string& msg) {
   last_error() = msg;

@@ -46,6 +46,13 @@
 #include "../../include/miniaero.h"
 #include "common.hpp"
 
+namespace ma {
+std::string& last_error() {
+  static thread_local std::string msg;
+  return msg;
+}
+}  // namespace ma
+
 namespace {
 
 struct FaceSet {
