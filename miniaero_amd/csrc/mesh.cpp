@@ -139,8 +139,8 @@ struct ma_mesh {
     c2[1] = -e2[0] * e3[2] + e3[0] * e2[2];
     c2[2] = e2[0] * e3[1] - e3[0] * e2[1];
     for (int d = 0; d < 3; ++d) a[d] = 0.5 * (c1[d] + c2[d]);
-    // tangent in the plane of the two largest... components: the first
-    // largest |a_i| and the component after the next
+    // tangent: zero in the component after the first largest |a_i| (i1),
+    // rotated in the plane of a_i1 and the component two after it (i3)
     const double m0 = std::fabs(a[0]), m1 = std::fabs(a[1]), m2 = std::fabs(a[2]);
     int i1 = 0;
     if (m1 > m0) i1 = 1;
