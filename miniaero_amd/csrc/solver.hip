@@ -29,7 +29,9 @@
 // results equal those of a one-thread CPU run of the reference.
 //
 // Layout in HBM (structure of arrays, one plane per component, so a
-// wavefront's 64 lanes read 64 consecutive doubles):
+// wavefront's 64 lanes read 64 consecutive doubles).  Cells are renumbered
+// into 4 x 8 x 8 bricks of the mesh lattice (one brick per flux tile) when the
+// mesh is a lattice; results go back to the caller in the caller's order.
 //   cells   vol[nc], xc[3][nc], U[2][5][nc] (U_n / U_acc ping-pong), R[5][nc],
 //           V[5][nc], grad[15][nc], phi[5][nc], slot[6][nc] (int32)
 //   faces   n[3][nf], frame[10][nf] (|n|, unit n, t, b: geometry-only part
@@ -58,6 +60,7 @@ namespace {
 constexpr int kMaxSlots = 6;  // hex8 cells (ElementTopoHexa8); checked at setup
 constexpr int kBlock = 256;
 constexpr int kTile = 256;    // cells per workgroup of the flux kernel
+constexpr int kXcds = 8;      // MI355X dies; workgroups are dealt to them round-robin
 
 struct DevMesh {
   int nc;    // owned cells
@@ -65,6 +68,7 @@ struct DevMesh {
   int nf;    // all faces
   const double* __restrict__ vol;
   const double* __restrict__ xc;   // 3 planes of nc
+  const int* __restrict__ org;     // device cell -> caller's cell index
   const int* __restrict__ slot;    // kMaxSlots planes of nc; entry = face*2 + side, -1 empty
   const double* __restrict__ fn;   // 3 planes of nf: area vector
   const double* __restrict__ ffr;  // 10 planes of nf: |n|, unit n, unit t, unit b
@@ -79,14 +83,17 @@ struct Inflow {
   double U[5];
 };
 
-// Tiles of the flux kernel: tile b holds cells [b*kTile, (b+1)*kTile) and the
-// internal faces faces[ptr[b] .. ptr[b+1]) that touch them (ascending).
+// Tiles of the flux kernel: tile b holds at most kTile cells, a brick of the
+// mesh lattice when there is one, and the internal faces
+// faces[ptr[b] .. ptr[b+1]) that touch them (ascending).
 struct TileMap {
   int ntiles;
+  int per_xcd;                    // tiles per XCD, ceil(ntiles / kXcds)
   int max_faces;                  // largest ptr[b+1] - ptr[b]
   const int* __restrict__ ptr;    // ntiles + 1
   const int* __restrict__ faces;  // device face indices
   const int* __restrict__ slot;   // kMaxSlots planes of nc, see k_flux
+  const int* __restrict__ cptr;   // ntiles + 1: cells of tile b are [cptr[b], cptr[b+1])
 };
 
 __device__ __forceinline__ void load5(const double* __restrict__ a, int n, int i, double* v) {
@@ -161,10 +168,11 @@ __global__ __launch_bounds__(kBlock) void k_init(DevMesh m, IcParams p, double* 
     ma::primitives(p.inflow, V);
     const double speed = sqrt(V[1] * V[1] + V[2] * V[2] + V[3] * V[3]);
     const double pr = V[0] * Rgas * V[4];
-    const double rho = V[0] * (1.0 + p.perturb * unif(p.seed, c, 0));
-    const double uu = V[1] + p.perturb * speed * unif(p.seed, c, 1);
-    const double vv = V[2] + p.perturb * speed * unif(p.seed, c, 2);
-    const double ww = V[3] + p.perturb * speed * unif(p.seed, c, 3);
+    const int oc = m.org[c];  // the draw is keyed by the caller's cell index
+    const double rho = V[0] * (1.0 + p.perturb * unif(p.seed, oc, 0));
+    const double uu = V[1] + p.perturb * speed * unif(p.seed, oc, 1);
+    const double vv = V[2] + p.perturb * speed * unif(p.seed, oc, 2);
+    const double ww = V[3] + p.perturb * speed * unif(p.seed, oc, 3);
     const double T = pr / (rho * Rgas);
     const double Cv = Rgas / (gamma - 1.0);
     u[0] = rho;
@@ -461,7 +469,10 @@ __global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const doub
                                                 const double* __restrict__ dt_dev, double beta,
                                                 Visc visc) {
   extern __shared__ double lds[];  // 5 planes of t.max_faces
-  const int tile = blockIdx.x;
+  // XCD-aware order: the workgroups an XCD receives (every kXcds-th) take a
+  // contiguous run of tiles, so neighbouring tiles share that XCD's L2.
+  const int tile = (blockIdx.x % kXcds) * t.per_xcd + blockIdx.x / kXcds;
+  if (tile >= t.ntiles) return;  // whole workgroup, before any barrier
   const int f0 = t.ptr[tile];
   const int nfl = t.ptr[tile + 1] - f0;
   const int mf = t.max_faces;
@@ -472,8 +483,8 @@ __global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const doub
     for (int k = 0; k < 5; ++k) lds[k * mf + i] = F[k];
   }
   __syncthreads();
-  const int c = tile * kTile + threadIdx.x;
-  if (c >= m.nc) return;
+  const int c = t.cptr[tile] + threadIdx.x;
+  if (c >= t.cptr[tile + 1]) return;
   const int nc = m.nc;
   const int nbf = m.nf - m.nif;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -617,6 +628,7 @@ struct ma_solver {
   hipStream_t stream = nullptr;
   DevMesh m{};
   TileMap t{};
+  std::vector<int> dev_of;  // caller's cell index -> device cell index
   double* bc = nullptr;  // boundary-face contributions, 5 planes of nbf
   std::vector<void*> allocs;
   size_t bytes = 0;
@@ -707,6 +719,87 @@ ma_status upload(ma_solver* s, T** dst, const std::vector<T>& h) {
   return MA_OK;
 }
 
+// Lattice coordinates of a structured hex mesh from its face connectivity:
+// every internal face links its two cells along the axis of the largest
+// component of its normal; on a structured mesh those links form one chain
+// per grid line, and a cell's position along its chain is its index on that
+// axis.  Returns false when the mesh is not a full i x j x k lattice.
+bool lattice_coords(const ma_mesh_data* md, std::vector<int> ijk[3], int dims[3]) {
+  const int nc = md->num_owned_cells;
+  const ma_faces& I = md->internal_faces;
+  std::vector<int> succ[3], haspred[3];
+  for (int a = 0; a < 3; ++a) {
+    succ[a].assign(nc, -1);
+    haspred[a].assign(nc, 0);
+  }
+  for (int i = 0; i < I.nfaces; ++i) {
+    const double* n = &I.face_normal[3 * i];
+    int a = 0;
+    if (std::fabs(n[1]) > std::fabs(n[a])) a = 1;
+    if (std::fabs(n[2]) > std::fabs(n[a])) a = 2;
+    int from = I.face_cell_conn[2 * i], to = I.face_cell_conn[2 * i + 1];
+    if (n[a] < 0) std::swap(from, to);
+    if (succ[a][from] != -1 || haspred[a][to]) return false;
+    succ[a][from] = to;
+    haspred[a][to] = 1;
+  }
+  for (int a = 0; a < 3; ++a) {
+    ijk[a].assign(nc, -1);
+    dims[a] = 0;
+    for (int c = 0; c < nc; ++c) {
+      if (haspred[a][c]) continue;
+      int k = 0;
+      for (int x = c; x != -1; x = succ[a][x]) {
+        if (ijk[a][x] != -1) return false;
+        ijk[a][x] = k++;
+      }
+      dims[a] = std::max(dims[a], k);
+    }
+    for (int c = 0; c < nc; ++c)
+      if (ijk[a][c] < 0) return false;  // a cycle
+  }
+  if ((long long)dims[0] * dims[1] * dims[2] != nc) return false;
+  std::vector<char> seen(nc, 0);
+  for (int c = 0; c < nc; ++c) {
+    const long long id = ((long long)ijk[0][c] * dims[1] + ijk[1][c]) * dims[2] + ijk[2][c];
+    if (id >= nc || seen[id]) return false;
+    seen[id] = 1;
+  }
+  return true;
+}
+
+// Device cell order: bricks of kBrick[0] x kBrick[1] x kBrick[2] lattice cells
+// (brick-lexicographic, last axis fastest, and the same inside a brick), so a
+// 256-cell tile is a compact block whose neighbours are mostly inside it.
+// Without a lattice the original order is kept.  Fills dev_of (original ->
+// device) and the cell range of every tile.
+void cell_order(const ma_mesh_data* md, std::vector<int>& dev_of, std::vector<int>& tile_cptr) {
+  const int nc = md->num_owned_cells;
+  static const int kBrick[3] = {4, 8, 8};
+  std::vector<int> ijk[3];
+  int dims[3];
+  dev_of.resize(nc);
+  tile_cptr.assign(1, 0);
+  if (!lattice_coords(md, ijk, dims)) {
+    for (int c = 0; c < nc; ++c) dev_of[c] = c;
+    for (int b = kTile; b < nc; b += kTile) tile_cptr.push_back(b);
+    tile_cptr.push_back(nc);
+    return;
+  }
+  std::vector<int> at((size_t)nc);
+  for (int c = 0; c < nc; ++c) at[((size_t)ijk[0][c] * dims[1] + ijk[1][c]) * dims[2] + ijk[2][c]] = c;
+  int d = 0;
+  for (int bi = 0; bi < dims[0]; bi += kBrick[0])
+    for (int bj = 0; bj < dims[1]; bj += kBrick[1])
+      for (int bk = 0; bk < dims[2]; bk += kBrick[2]) {
+        for (int i = bi; i < std::min(bi + kBrick[0], dims[0]); ++i)
+          for (int j = bj; j < std::min(bj + kBrick[1], dims[1]); ++j)
+            for (int k = bk; k < std::min(bk + kBrick[2], dims[2]); ++k)
+              dev_of[at[((size_t)i * dims[1] + j) * dims[2] + k]] = d++;
+        tile_cptr.push_back(d);
+      }
+}
+
 ma_status build(ma_solver* s, const ma_mesh_data* md) {
   const int nc = md->num_owned_cells;
   const ma_faces& I = md->internal_faces;
@@ -721,19 +814,40 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
     if (st) return st;
     nbf += G.faces.nfaces;
   }
-  if ((long long)I.nfaces + nbf > (1ll << 30))
-    return ma::fail(MA_ERR_UNSUPPORTED, "more than 2^30 faces");
+  if ((long long)I.nfaces + nbf > (1ll << 28))
+    return ma::fail(MA_ERR_UNSUPPORTED, "more than 2^28 faces");
   const int nif = I.nfaces, nf = nif + (int)nbf;
 
-  // Device face order: internal faces in processing order, then every
-  // boundary group in order.
+  // Cell order on the device, and its inverse.
+  std::vector<int>& dev_of = s->dev_of;
+  std::vector<int> tile_cptr;
+  cell_order(md, dev_of, tile_cptr);
+  std::vector<int> org_of(nc);
+  for (int c = 0; c < nc; ++c) org_of[dev_of[c]] = c;
+  const int ntiles = (int)tile_cptr.size() - 1;
+
+  // Summation order: internal faces in processing order (rank[i] = position
+  // of face i in it).  Device face order: internal faces sorted by the device
+  // index of their left cell (stable in processing order), then the boundary
+  // groups in order.  Memory order does not change any sum.
   const std::vector<int> perm = processing_order(I, nc);
+  std::vector<int> dface(nif);  // device slot -> original internal face
+  {
+    std::vector<int> start((size_t)nc + 1, 0);
+    for (int i = 0; i < nif; ++i) start[dev_of[I.face_cell_conn[2 * i]] + 1]++;
+    for (int c = 0; c < nc; ++c) start[c + 1] += start[c];
+    for (int ii = 0; ii < nif; ++ii) {
+      const int i = perm[ii];
+      dface[start[dev_of[I.face_cell_conn[2 * i]]]++] = i;
+    }
+  }
+  std::vector<int> devf_of(nif);  // original internal face -> device slot
+  for (int g = 0; g < nif; ++g) devf_of[dface[g]] = g;
+
   std::vector<double> fn((size_t)3 * nf), ffr((size_t)10 * nf), fx((size_t)3 * nf);
   std::vector<int> fl(nf), fr(nf);
   std::vector<signed char> fkind(nf, 0);
   std::vector<double> fwall(nf, 0.0);
-  std::vector<int> slot((size_t)kMaxSlots * nc, -1);
-  std::vector<unsigned char> used(nc, 0);
   auto put = [&](int f, const ma_faces& F, int i) {
     const ma::FaceFrame fr = ma::make_frame(&F.face_normal[3 * i], &F.face_tangent[3 * i],
                                             &F.face_binormal[3 * i]);
@@ -746,45 +860,59 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
       ffr[(size_t)(7 + d) * nf + f] = fr.bu[d];
     }
   };
-  auto add_slot = [&](int c, int entry) -> bool {
-    if (used[c] >= kMaxSlots) return false;
-    slot[(size_t)used[c] * nc + c] = entry;
-    used[c]++;
+  for (int g = 0; g < nif; ++g) {
+    const int i = dface[g];
+    put(g, I, i);
+    fl[g] = dev_of[I.face_cell_conn[2 * i]];
+    fr[g] = dev_of[I.face_cell_conn[2 * i + 1]];
+  }
+  {
+    int f = nif;
+    for (int g = 0; g < md->num_boundary_groups; ++g) {
+      const ma_bc_group& G = md->boundary_faces[g];
+      for (int i = 0; i < G.faces.nfaces; ++i, ++f) {
+        put(f, G.faces, i);
+        const int c = G.faces.face_cell_conn[2 * i];
+        fl[f] = dev_of[c];
+        fr[f] = -1;
+        fkind[f] = (signed char)G.kind;
+        // NoSlip_BC.h:124-129: 1 / sqrt(sum_d pow(x_f - x_c, 2))
+        double dist = 0;
+        for (int d = 0; d < 3; ++d)
+          dist += std::pow(G.faces.coordinates[3 * i + d] - md->cell_coordinates[3 * c + d], 2);
+        fwall[f] = 1.0 / std::sqrt(dist);
+      }
+    }
+  }
+
+  // Per-cell face lists in summation order: internal faces by processing
+  // rank, then boundary faces in group order.  Entry = device face * 2 + side.
+  std::vector<int> slot((size_t)kMaxSlots * nc, -1);
+  std::vector<unsigned char> used(nc, 0);
+  auto add_slot = [&](int dc, int entry) -> bool {
+    if (used[dc] >= kMaxSlots) return false;
+    slot[(size_t)used[dc] * nc + dc] = entry;
+    used[dc]++;
     return true;
   };
   for (int ii = 0; ii < nif; ++ii) {
-    const int i = perm[ii];
-    put(ii, I, i);
-    fl[ii] = I.face_cell_conn[2 * i];
-    fr[ii] = I.face_cell_conn[2 * i + 1];
-    if (!add_slot(fl[ii], ii * 2) || !add_slot(fr[ii], ii * 2 + 1))
+    const int i = perm[ii], g = devf_of[i];
+    if (!add_slot(fl[g], g * 2) || !add_slot(fr[g], g * 2 + 1))
       return ma::fail(MA_ERR_UNSUPPORTED, "a cell has more than 6 faces (hex8 meshes only)");
   }
-  int f = nif;
-  for (int g = 0; g < md->num_boundary_groups; ++g) {
-    const ma_bc_group& G = md->boundary_faces[g];
-    for (int i = 0; i < G.faces.nfaces; ++i, ++f) {
-      put(f, G.faces, i);
-      const int c = G.faces.face_cell_conn[2 * i];
-      fl[f] = c;
-      fr[f] = -1;
-      fkind[f] = (signed char)G.kind;
-      // NoSlip_BC.h:124-129: 1 / sqrt(sum_d pow(x_f - x_c, 2))
-      double dist = 0;
-      for (int d = 0; d < 3; ++d)
-        dist += std::pow(G.faces.coordinates[3 * i + d] - md->cell_coordinates[3 * c + d], 2);
-      fwall[f] = 1.0 / std::sqrt(dist);
-      if (!add_slot(c, f * 2))
-        return ma::fail(MA_ERR_UNSUPPORTED, "a cell has more than 6 faces (hex8 meshes only)");
-    }
-  }
+  for (int f = nif; f < nf; ++f)
+    if (!add_slot(fl[f], f * 2))
+      return ma::fail(MA_ERR_UNSUPPORTED, "a cell has more than 6 faces (hex8 meshes only)");
+
   // Tiles of the flux kernel.  Every internal face is listed in the tile of
   // its left cell and, when different, in the tile of its right cell; faces
   // are appended in ascending device order, so each list is sorted.
-  const int ntiles = (nc + kTile - 1) / kTile;
+  std::vector<int> tile_of(nc);
+  for (int b = 0; b < ntiles; ++b)
+    for (int d = tile_cptr[b]; d < tile_cptr[b + 1]; ++d) tile_of[d] = b;
   std::vector<int> tptr((size_t)ntiles + 1, 0);
   for (int g = 0; g < nif; ++g) {
-    const int tl = fl[g] / kTile, tr = fr[g] / kTile;
+    const int tl = tile_of[fl[g]], tr = tile_of[fr[g]];
     tptr[tl + 1]++;
     if (tr != tl) tptr[tr + 1]++;
   }
@@ -793,7 +921,7 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
   {
     std::vector<int> fill(tptr.begin(), tptr.end() - 1);
     for (int g = 0; g < nif; ++g) {
-      const int tl = fl[g] / kTile, tr = fr[g] / kTile;
+      const int tl = tile_of[fl[g]], tr = tile_of[fr[g]];
       locl[g] = fill[tl] - tptr[tl];
       tfaces[fill[tl]++] = g;
       if (tr != tl) {
@@ -818,26 +946,60 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
           g < nif ? (((side ? locr[g] : locl[g]) << 3) | side) : (((g - nif) << 3) | (3 + fkind[g]));
     }
 
-  std::vector<double> vol(md->cell_volumes, md->cell_volumes + nc);
-  std::vector<double> xc((size_t)3 * nc);
-  for (int c = 0; c < nc; ++c)
-    for (int d = 0; d < 3; ++d) xc[(size_t)d * nc + c] = md->cell_coordinates[3 * c + d];
+  // Every index a kernel will use, checked on the host before any upload.
+  {
+    std::vector<char> hit(nc, 0);
+    for (int c = 0; c < nc; ++c) {
+      if (dev_of[c] < 0 || dev_of[c] >= nc || hit[dev_of[c]]) return ma::fail(MA_ERR_INVALID, "layout: cell order");
+      hit[dev_of[c]] = 1;
+    }
+    if (tile_cptr.front() != 0 || tile_cptr.back() != nc) return ma::fail(MA_ERR_INVALID, "layout: tile cells");
+    for (int b = 0; b < ntiles; ++b) {
+      const int n = tile_cptr[b + 1] - tile_cptr[b];
+      if (n < 1 || n > kTile) return ma::fail(MA_ERR_INVALID, "layout: tile size");
+      for (int q = tptr[b]; q < tptr[b + 1]; ++q)
+        if (tfaces[q] < 0 || tfaces[q] >= nif) return ma::fail(MA_ERR_INVALID, "layout: tile face");
+    }
+    for (int f = 0; f < nf; ++f)
+      if (fl[f] < 0 || fl[f] >= nc || (f < nif && (fr[f] < 0 || fr[f] >= nc)))
+        return ma::fail(MA_ERR_INVALID, "layout: face cells");
+    for (int c = 0; c < nc; ++c) {
+      const int b = tile_of[c], nloc = tptr[b + 1] - tptr[b];
+      for (int q = 0; q < kMaxSlots; ++q) {
+        const int e = tslot[(size_t)q * nc + c], g = slot[(size_t)q * nc + c];
+        if ((e < 0) != (g < 0)) return ma::fail(MA_ERR_INVALID, "layout: slot tables");
+        if (e < 0) continue;
+        if ((g >> 1) >= nf) return ma::fail(MA_ERR_INVALID, "layout: slot face");
+        if ((e & 7) < 2 ? (e >> 3) >= nloc || tfaces[tptr[b] + (e >> 3)] != (g >> 1)
+                        : (e >> 3) >= nbf)
+          return ma::fail(MA_ERR_INVALID, "layout: tile slot");
+      }
+    }
+  }
+
+  std::vector<double> vol(nc), xc((size_t)3 * nc);
+  for (int c = 0; c < nc; ++c) {
+    const int d = dev_of[c];
+    vol[d] = md->cell_volumes[c];
+    for (int k = 0; k < 3; ++k) xc[(size_t)k * nc + d] = md->cell_coordinates[3 * c + k];
+  }
 
   DevMesh& m = s->m;
   m.nc = nc;
   m.nif = nif;
   m.nf = nf;
   double *dvol, *dxc, *dfn, *dffr, *dfx, *dfw;
-  int *dslot, *dfl, *dfr;
+  int *dslot, *dfl, *dfr, *dorg;
   signed char* dkind;
   if ((st = upload(s, &dvol, vol)) || (st = upload(s, &dxc, xc)) || (st = upload(s, &dslot, slot)) ||
-      (st = upload(s, &dfn, fn)) || (st = upload(s, &dffr, ffr)) ||
+      (st = upload(s, &dfn, fn)) || (st = upload(s, &dffr, ffr)) || (st = upload(s, &dorg, org_of)) ||
       (st = upload(s, &dfx, fx)) || (st = upload(s, &dfl, fl)) || (st = upload(s, &dfr, fr)) ||
       (st = upload(s, &dkind, fkind)) || (st = upload(s, &dfw, fwall)))
     return st;
   m.vol = dvol;
   m.xc = dxc;
   m.slot = dslot;
+  m.org = dorg;
   m.fn = dfn;
   m.ffr = dffr;
   m.fx = dfx;
@@ -846,15 +1008,17 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
   m.fkind = dkind;
   m.fwall = dfw;
 
-  int *dtptr, *dtfaces, *dtslot;
+  int *dtptr, *dtfaces, *dtslot, *dtcptr;
   if ((st = upload(s, &dtptr, tptr)) || (st = upload(s, &dtfaces, tfaces)) || (st = upload(s, &dtslot, tslot)) ||
-      (st = s->alloc(&s->bc, (size_t)5 * std::max<long long>(nbf, 1))))
+      (st = upload(s, &dtcptr, tile_cptr)) || (st = s->alloc(&s->bc, (size_t)5 * std::max<long long>(nbf, 1))))
     return st;
   s->t.ntiles = ntiles;
+  s->t.per_xcd = (ntiles + kXcds - 1) / kXcds;
   s->t.max_faces = max_faces;
   s->t.ptr = dtptr;
   s->t.faces = dtfaces;
   s->t.slot = dtslot;
+  s->t.cptr = dtcptr;
 
   const size_t n5 = (size_t)5 * nc;
   if ((st = s->alloc(&s->U[0], n5)) || (st = s->alloc(&s->U[1], n5)) || (st = s->alloc(&s->R, n5)) ||
@@ -899,7 +1063,7 @@ void run_stage(ma_solver* s, int irk) {
   }
   const size_t lds = (size_t)s->t.max_faces * 5 * sizeof(double);
   s->launch("k_flux", [&] {
-    hipLaunchKernelGGL((k_flux<SO, Visc>), dim3(s->t.ntiles), dim3(kTile), lds, s->stream, m, s->t,
+    hipLaunchKernelGGL((k_flux<SO, Visc>), dim3(s->t.per_xcd * kXcds), dim3(kTile), lds, s->stream, m, s->t,
                        s->V, s->grad, s->phi, s->bc, acc_in, Uacc, s->R, s->dt_dev, beta[irk], visc);
   });
 }
@@ -946,8 +1110,10 @@ ma_status download_aos(ma_solver* s, const double* dsrc, int ncomp, double* host
   std::vector<double> tmp((size_t)ncomp * nc);
   MA_HIP(hipMemcpyAsync(tmp.data(), dsrc, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
   MA_HIP(hipStreamSynchronize(s->stream));
-  for (int c = 0; c < nc; ++c)
-    for (int k = 0; k < ncomp; ++k) host[(size_t)c * ncomp + k] = tmp[(size_t)k * nc + c];
+  for (int c = 0; c < nc; ++c) {
+    const int d = s->dev_of[c];
+    for (int k = 0; k < ncomp; ++k) host[(size_t)c * ncomp + k] = tmp[(size_t)k * nc + d];
+  }
   return MA_OK;
 }
 
@@ -1024,7 +1190,7 @@ extern "C" ma_status ma_solver_set_state(ma_solver* s, const double* U) {
   const int nc = s->m.nc;
   std::vector<double> tmp((size_t)5 * nc);
   for (int c = 0; c < nc; ++c)
-    for (int k = 0; k < 5; ++k) tmp[(size_t)k * nc + c] = U[(size_t)c * 5 + k];
+    for (int k = 0; k < 5; ++k) tmp[(size_t)k * nc + s->dev_of[c]] = U[(size_t)c * 5 + k];
   s->cur = 0;
   MA_HIP(hipMemcpyAsync(s->U[0], tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
   MA_HIP(hipMemsetAsync(s->R, 0, tmp.size() * sizeof(double), s->stream));
