@@ -125,6 +125,13 @@ ma_status ma_mesh_generate(const ma_options* opt, ma_mesh** out);
 ma_status ma_mesh_view(const ma_mesh* mesh, ma_mesh_data* view);
 void ma_mesh_destroy(ma_mesh* mesh);
 
+/* Pre-flight check of a mesh, host only (no GPU needed): validates the
+ * arrays and builds the device layout ma_solver_create would use.  stats[0..2]
+ * = lattice dimensions found from the face connectivity (0 when the mesh is
+ * not a lattice), [3] = flux tiles, [4] = most internal faces in one tile,
+ * [5] = tile face entries in total.  No reference counterpart. */
+ma_status ma_mesh_plan(const ma_mesh_data* mesh, long long* stats);
+
 /* ---- solver (replaces TimeSolverExplicitRK4<Device>, TimeSolverExplicitRK4.h:160-539) ---- */
 
 /* TimeSolverExplicitRK4(MeshData&, const Options&) (TimeSolverExplicitRK4.h:164,179-196):

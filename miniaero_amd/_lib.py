@@ -31,7 +31,7 @@ EXPORTS = (
     "ma_solver_synchronize", "ma_solver_get_state", "ma_solver_get_residual",
     "ma_solver_get_gradients", "ma_solver_get_time", "ma_solver_check_finite",
     "ma_solver_stream", "ma_solver_destroy", "ma_solver_profile", "ma_solver_sizes",
-    "ma_measure_copy",
+    "ma_measure_copy", "ma_mesh_plan",
 )
 
 
@@ -134,6 +134,7 @@ def load() -> ctypes.CDLL:
     L.ma_solver_profile.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _I, _D, ctypes.c_int]
     L.ma_solver_sizes.argtypes = [P, ctypes.POINTER(ctypes.c_longlong)]
     L.ma_measure_copy.argtypes = [ctypes.c_longlong, ctypes.c_int, _D]
+    L.ma_mesh_plan.argtypes = [ctypes.POINTER(ma_mesh_data), ctypes.POINTER(ctypes.c_longlong)]
     _lib = L
     return L
 

@@ -773,14 +773,15 @@ bool lattice_coords(const ma_mesh_data* md, std::vector<int> ijk[3], int dims[3]
 // 256-cell tile is a compact block whose neighbours are mostly inside it.
 // Without a lattice the original order is kept.  Fills dev_of (original ->
 // device) and the cell range of every tile.
-void cell_order(const ma_mesh_data* md, std::vector<int>& dev_of, std::vector<int>& tile_cptr) {
+void cell_order(const ma_mesh_data* md, std::vector<int>& dev_of, std::vector<int>& tile_cptr,
+                int dims[3]) {
   const int nc = md->num_owned_cells;
   static const int kBrick[3] = {4, 8, 8};
   std::vector<int> ijk[3];
-  int dims[3];
   dev_of.resize(nc);
   tile_cptr.assign(1, 0);
   if (!lattice_coords(md, ijk, dims)) {
+    dims[0] = dims[1] = dims[2] = 0;
     for (int c = 0; c < nc; ++c) dev_of[c] = c;
     for (int b = kTile; b < nc; b += kTile) tile_cptr.push_back(b);
     tile_cptr.push_back(nc);
@@ -800,8 +801,18 @@ void cell_order(const ma_mesh_data* md, std::vector<int>& dev_of, std::vector<in
       }
 }
 
-ma_status build(ma_solver* s, const ma_mesh_data* md) {
+// Everything the device needs, computed on the host (no GPU involved).
+struct Layout {
+  int nc = 0, nif = 0, nf = 0, nbf = 0, ntiles = 0, max_faces = 1;
+  int dims[3] = {0, 0, 0};  // mesh lattice, zeros when there is none
+  std::vector<int> dev_of, org_of, tile_cptr, tptr, tfaces, slot, tslot, fl, fr;
+  std::vector<double> fn, ffr, fx, fwall, vol, xc;
+  std::vector<signed char> fkind;
+};
+
+ma_status plan_layout(const ma_mesh_data* md, Layout& L) {
   const int nc = md->num_owned_cells;
+  L.nc = nc;
   const ma_faces& I = md->internal_faces;
   ma_status st = check_faces(I, nc, true, "internal_faces");
   if (st) return st;
@@ -817,12 +828,16 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
   if ((long long)I.nfaces + nbf > (1ll << 28))
     return ma::fail(MA_ERR_UNSUPPORTED, "more than 2^28 faces");
   const int nif = I.nfaces, nf = nif + (int)nbf;
+  L.nif = nif;
+  L.nf = nf;
+  L.nbf = (int)nbf;
 
   // Cell order on the device, and its inverse.
-  std::vector<int>& dev_of = s->dev_of;
-  std::vector<int> tile_cptr;
-  cell_order(md, dev_of, tile_cptr);
-  std::vector<int> org_of(nc);
+  std::vector<int>& dev_of = L.dev_of;
+  std::vector<int>& tile_cptr = L.tile_cptr;
+  cell_order(md, dev_of, tile_cptr, L.dims);
+  std::vector<int>& org_of = L.org_of;
+  org_of.assign(nc, 0);
   for (int c = 0; c < nc; ++c) org_of[dev_of[c]] = c;
   const int ntiles = (int)tile_cptr.size() - 1;
 
@@ -844,10 +859,20 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
   std::vector<int> devf_of(nif);  // original internal face -> device slot
   for (int g = 0; g < nif; ++g) devf_of[dface[g]] = g;
 
-  std::vector<double> fn((size_t)3 * nf), ffr((size_t)10 * nf), fx((size_t)3 * nf);
-  std::vector<int> fl(nf), fr(nf);
-  std::vector<signed char> fkind(nf, 0);
-  std::vector<double> fwall(nf, 0.0);
+  std::vector<double>& fn = L.fn;
+  std::vector<double>& ffr = L.ffr;
+  std::vector<double>& fx = L.fx;
+  std::vector<double>& fwall = L.fwall;
+  std::vector<int>& fl = L.fl;
+  std::vector<int>& fr = L.fr;
+  std::vector<signed char>& fkind = L.fkind;
+  fn.assign((size_t)3 * nf, 0.0);
+  ffr.assign((size_t)10 * nf, 0.0);
+  fx.assign((size_t)3 * nf, 0.0);
+  fwall.assign(nf, 0.0);
+  fl.assign(nf, 0);
+  fr.assign(nf, 0);
+  fkind.assign(nf, 0);
   auto put = [&](int f, const ma_faces& F, int i) {
     const ma::FaceFrame fr = ma::make_frame(&F.face_normal[3 * i], &F.face_tangent[3 * i],
                                             &F.face_binormal[3 * i]);
@@ -887,7 +912,8 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
 
   // Per-cell face lists in summation order: internal faces by processing
   // rank, then boundary faces in group order.  Entry = device face * 2 + side.
-  std::vector<int> slot((size_t)kMaxSlots * nc, -1);
+  std::vector<int>& slot = L.slot;
+  slot.assign((size_t)kMaxSlots * nc, -1);
   std::vector<unsigned char> used(nc, 0);
   auto add_slot = [&](int dc, int entry) -> bool {
     if (used[dc] >= kMaxSlots) return false;
@@ -910,14 +936,17 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
   std::vector<int> tile_of(nc);
   for (int b = 0; b < ntiles; ++b)
     for (int d = tile_cptr[b]; d < tile_cptr[b + 1]; ++d) tile_of[d] = b;
-  std::vector<int> tptr((size_t)ntiles + 1, 0);
+  std::vector<int>& tptr = L.tptr;
+  tptr.assign((size_t)ntiles + 1, 0);
   for (int g = 0; g < nif; ++g) {
     const int tl = tile_of[fl[g]], tr = tile_of[fr[g]];
     tptr[tl + 1]++;
     if (tr != tl) tptr[tr + 1]++;
   }
   for (int b = 0; b < ntiles; ++b) tptr[b + 1] += tptr[b];
-  std::vector<int> tfaces((size_t)std::max(1, tptr[ntiles])), locl(nif), locr(nif);
+  std::vector<int>& tfaces = L.tfaces;
+  tfaces.assign((size_t)std::max(1, tptr[ntiles]), 0);
+  std::vector<int> locl(nif), locr(nif);
   {
     std::vector<int> fill(tptr.begin(), tptr.end() - 1);
     for (int g = 0; g < nif; ++g) {
@@ -936,7 +965,8 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
   for (int b = 0; b < ntiles; ++b) max_faces = std::max(max_faces, tptr[b + 1] - tptr[b]);
   if ((size_t)max_faces * 5 * sizeof(double) > 65536)
     return ma::fail(MA_ERR_UNSUPPORTED, "too many faces per cell tile for the flux kernel's LDS");
-  std::vector<int> tslot((size_t)kMaxSlots * nc, -1);
+  std::vector<int>& tslot = L.tslot;
+  tslot.assign((size_t)kMaxSlots * nc, -1);
   for (int c = 0; c < nc; ++c)
     for (int q = 0; q < kMaxSlots; ++q) {
       const int e = slot[(size_t)q * nc + c];
@@ -977,13 +1007,25 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
     }
   }
 
-  std::vector<double> vol(nc), xc((size_t)3 * nc);
+  L.vol.assign(nc, 0.0);
+  L.xc.assign((size_t)3 * nc, 0.0);
   for (int c = 0; c < nc; ++c) {
     const int d = dev_of[c];
-    vol[d] = md->cell_volumes[c];
-    for (int k = 0; k < 3; ++k) xc[(size_t)k * nc + d] = md->cell_coordinates[3 * c + k];
+    L.vol[d] = md->cell_volumes[c];
+    for (int k = 0; k < 3; ++k) L.xc[(size_t)k * nc + d] = md->cell_coordinates[3 * c + k];
   }
+  L.ntiles = ntiles;
+  L.max_faces = max_faces;
+  return MA_OK;
+}
 
+ma_status build(ma_solver* s, const ma_mesh_data* md) {
+  Layout L;
+  ma_status st = plan_layout(md, L);
+  if (st) return st;
+  const int nc = L.nc, nif = L.nif, nf = L.nf;
+  const long long nbf = L.nbf;
+  s->dev_of.swap(L.dev_of);
   DevMesh& m = s->m;
   m.nc = nc;
   m.nif = nif;
@@ -991,10 +1033,10 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
   double *dvol, *dxc, *dfn, *dffr, *dfx, *dfw;
   int *dslot, *dfl, *dfr, *dorg;
   signed char* dkind;
-  if ((st = upload(s, &dvol, vol)) || (st = upload(s, &dxc, xc)) || (st = upload(s, &dslot, slot)) ||
-      (st = upload(s, &dfn, fn)) || (st = upload(s, &dffr, ffr)) || (st = upload(s, &dorg, org_of)) ||
-      (st = upload(s, &dfx, fx)) || (st = upload(s, &dfl, fl)) || (st = upload(s, &dfr, fr)) ||
-      (st = upload(s, &dkind, fkind)) || (st = upload(s, &dfw, fwall)))
+  if ((st = upload(s, &dvol, L.vol)) || (st = upload(s, &dxc, L.xc)) || (st = upload(s, &dslot, L.slot)) ||
+      (st = upload(s, &dfn, L.fn)) || (st = upload(s, &dffr, L.ffr)) || (st = upload(s, &dorg, L.org_of)) ||
+      (st = upload(s, &dfx, L.fx)) || (st = upload(s, &dfl, L.fl)) || (st = upload(s, &dfr, L.fr)) ||
+      (st = upload(s, &dkind, L.fkind)) || (st = upload(s, &dfw, L.fwall)))
     return st;
   m.vol = dvol;
   m.xc = dxc;
@@ -1009,12 +1051,12 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
   m.fwall = dfw;
 
   int *dtptr, *dtfaces, *dtslot, *dtcptr;
-  if ((st = upload(s, &dtptr, tptr)) || (st = upload(s, &dtfaces, tfaces)) || (st = upload(s, &dtslot, tslot)) ||
-      (st = upload(s, &dtcptr, tile_cptr)) || (st = s->alloc(&s->bc, (size_t)5 * std::max<long long>(nbf, 1))))
+  if ((st = upload(s, &dtptr, L.tptr)) || (st = upload(s, &dtfaces, L.tfaces)) || (st = upload(s, &dtslot, L.tslot)) ||
+      (st = upload(s, &dtcptr, L.tile_cptr)) || (st = s->alloc(&s->bc, (size_t)5 * std::max<long long>(nbf, 1))))
     return st;
-  s->t.ntiles = ntiles;
-  s->t.per_xcd = (ntiles + kXcds - 1) / kXcds;
-  s->t.max_faces = max_faces;
+  s->t.ntiles = L.ntiles;
+  s->t.per_xcd = (L.ntiles + kXcds - 1) / kXcds;
+  s->t.max_faces = L.max_faces;
   s->t.ptr = dtptr;
   s->t.faces = dtfaces;
   s->t.slot = dtslot;
@@ -1120,6 +1162,31 @@ ma_status download_aos(ma_solver* s, const double* dsrc, int ncomp, double* host
 }  // namespace
 
 extern "C" const char* ma_last_error(void) { return ma::last_error().c_str(); }
+
+extern "C" ma_status ma_mesh_plan(const ma_mesh_data* md, long long* stats) {
+  if (!md || !stats) return ma::fail(MA_ERR_INVALID, "ma_mesh_plan: null argument");
+  if (md->num_ghosts != 0)
+    return ma::fail(MA_ERR_UNSUPPORTED, "ghost cells (multi-rank meshes) are not supported on one GPU");
+  if (md->num_owned_cells <= 0 || !md->cell_coordinates || !md->cell_volumes)
+    return ma::fail(MA_ERR_INVALID, "ma_mesh_plan: empty or null cell arrays");
+  if (md->num_boundary_groups < 0 || (md->num_boundary_groups > 0 && !md->boundary_faces))
+    return ma::fail(MA_ERR_INVALID, "ma_mesh_plan: bad boundary group list");
+  Layout L;
+  ma_status st = MA_OK;
+  try {
+    st = plan_layout(md, L);
+  } catch (const std::bad_alloc&) {
+    st = ma::fail(MA_ERR_NOMEM, "ma_mesh_plan: out of host memory");
+  }
+  if (st) return st;
+  stats[0] = L.dims[0];
+  stats[1] = L.dims[1];
+  stats[2] = L.dims[2];
+  stats[3] = L.ntiles;
+  stats[4] = L.max_faces;
+  stats[5] = L.tptr.empty() ? 0 : L.tptr.back();
+  return MA_OK;
+}
 
 extern "C" ma_status ma_solver_create(const ma_mesh_data* md, const ma_options* opt, ma_solver** out) {
   if (!md || !opt || !out) return ma::fail(MA_ERR_INVALID, "ma_solver_create: null argument");

@@ -65,6 +65,14 @@ class MeshData:
     boundary_faces: list = field(default_factory=list)  # [(name, Faces)]
     num_ghosts: int = 0
 
+    def plan(self) -> dict:
+        """Host-only pre-flight check (ma_mesh_plan): validates the arrays and
+        reports the device layout the solver would build."""
+        st = (ctypes.c_longlong * 6)()
+        _lib.check(_lib.load().ma_mesh_plan(ctypes.byref(self.to_c()), st))
+        return dict(lattice=(st[0], st[1], st[2]), tiles=st[3], max_tile_faces=st[4],
+                    tile_face_entries=st[5])
+
     def to_c(self) -> _lib.ma_mesh_data:
         m = _lib.ma_mesh_data()
         m.num_owned_cells = int(self.num_owned_cells)

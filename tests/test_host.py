@@ -106,3 +106,62 @@ def test_cpp_driver_reports_missing_input(tmp_path):
     assert os.path.exists(exe), "bin/miniaero not built"
     r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "miniaero.inp not found" in r.stderr
+
+
+@pytest.mark.parametrize("shape", [(256 // 8, 128 // 8, 64 // 8), (7, 9, 10), (1, 1, 1), (1, 3, 1)])
+def test_layout_plan_finds_lattice_bricks(shape):
+    nx, ny, nz = shape
+    md = M.Parallel3DMesh(nx, ny, nz, 1.0, 0.5, 0.25, 1, options=M.Options(y_stretch=1.05)).fillMeshData()
+    p = md.plan()
+    assert p["lattice"] == (nx, ny, nz)
+    ceil = lambda a, b: -(-a // b)
+    assert p["tiles"] == ceil(nx, 4) * ceil(ny, 8) * ceil(nz, 8)
+    nif = md.internal_faces.nfaces_
+    assert nif <= p["tile_face_entries"] <= 2 * nif
+    assert p["max_tile_faces"] <= 6 * 256
+
+
+def test_layout_plan_on_ramp_and_relabelled_cells():
+    md = M.Parallel3DMesh(16, 8, 4, 2.0, 2.0, 1.0, 2, 30.0).fillMeshData()
+    assert md.plan()["lattice"] == (16, 8, 4)
+    # relabel the cells at random: the lattice comes from the connectivity
+    rng = np.random.default_rng(5)
+    nc = md.num_owned_cells
+    new = rng.permutation(nc)           # old -> new
+    old = np.argsort(new)
+    def relabel(F):
+        conn = F.face_cell_conn.copy()
+        m = conn >= 0
+        conn[m] = new[conn[m]]
+        return M.Faces(F.coordinates, F.face_normal, F.face_tangent, F.face_binormal, conn)
+    md2 = M.MeshData(nc, md.cell_coordinates[old], md.cell_volumes[old], relabel(md.internal_faces),
+                     [(n, relabel(F)) for n, F in md.boundary_faces])
+    assert md2.plan()["lattice"] == (16, 8, 4)
+
+
+def test_layout_plan_rejects_bad_meshes():
+    md = M.Parallel3DMesh(3, 3, 3, 1.0, 1.0, 1.0, 0).fillMeshData()
+    F = md.internal_faces
+    bad = F.face_cell_conn.copy()
+    bad[5, 1] = md.num_owned_cells
+    with pytest.raises(M.MiniAeroError) as e:
+        M.MeshData(md.num_owned_cells, md.cell_coordinates, md.cell_volumes,
+                   M.Faces(F.coordinates, F.face_normal, F.face_tangent, F.face_binormal, bad),
+                   md.boundary_faces).plan()
+    assert e.value.status == _lib.MA_ERR_INVALID
+    perm = F.permute_vector.copy()
+    perm[0] = perm[1]
+    with pytest.raises(M.MiniAeroError) as e:
+        M.MeshData(md.num_owned_cells, md.cell_coordinates, md.cell_volumes,
+                   M.Faces(F.coordinates, F.face_normal, F.face_tangent, F.face_binormal,
+                           F.face_cell_conn, permute_vector=perm), md.boundary_faces).plan()
+    assert e.value.status == _lib.MA_ERR_INVALID
+    # a seventh face on one cell
+    extra = md.boundary_faces + [("Extrapolate", md.boundary_faces[0][1])]
+    with pytest.raises(M.MiniAeroError) as e:
+        M.MeshData(md.num_owned_cells, md.cell_coordinates, md.cell_volumes, F, extra).plan()
+    assert e.value.status == _lib.MA_ERR_UNSUPPORTED
+    with pytest.raises(M.MiniAeroError) as e:
+        M.MeshData(md.num_owned_cells, md.cell_coordinates, md.cell_volumes, F, md.boundary_faces,
+                   num_ghosts=4).plan()
+    assert e.value.status == _lib.MA_ERR_UNSUPPORTED
