@@ -92,6 +92,10 @@ def main() -> int:
     ap.add_argument("--workload", default="flat_plate_c2")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", type=int, default=None,
+                    help="override the workload's arithmetic (0 FP64, 1 FP32 flux/gradient/limiter)")
+    ap.add_argument("--compare-fp64", action="store_true",
+                    help="also run the workload in FP64 and report the per-variable relative L2 error")
     ap.add_argument("--copy-ceiling", action="store_true",
                     help="only measure the plain copy kernel (1 GiB, read + write) and exit")
     args = ap.parse_args()
@@ -112,6 +116,8 @@ def main() -> int:
         return 0
 
     opts = WORKLOADS[args.workload]()
+    if args.precision is not None:
+        opts = opts.replace(precision=args.precision)
     t0 = time.perf_counter()
     md = M.Parallel3DMesh(opts.nx, opts.ny, opts.nz, opts.lx, opts.ly, opts.lz, opts.problem_type,
                           opts.angle, options=opts).fillMeshData()
@@ -135,6 +141,25 @@ def main() -> int:
         return 1
     value = nc * 4 * args.steps / elapsed
     print(f"[bench] {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    precision_error = None
+    if args.compare_fp64 and opts.precision != 0:
+        # same steps from the same start in FP64; error of the timed run's state
+        import numpy as np
+        U = solver.state()
+        ref = M.TimeSolverExplicitRK4(md, opts.replace(precision=0))
+        ref.initialize()
+        ref.step(args.warmup + args.steps)
+        R = ref.state()
+        ref.close()
+        tot = np.linalg.norm(R)
+        names = ("rho", "rho_u", "rho_v", "rho_w", "rho_E")
+        precision_error = {
+            "metric": "relative L2 vs FP64 after warmup+steps, floor 1e-8*||U||",
+            "steps": args.warmup + args.steps,
+            **{n: float(np.linalg.norm(U[:, k] - R[:, k]) / max(np.linalg.norm(R[:, k]), 1e-8 * tot))
+               for k, n in enumerate(names)}}
+        print(f"[bench] precision error: {json.dumps(precision_error)}", file=sys.stderr, flush=True)
 
     prof = solver.profile(args.profile_steps)
     so = bool(opts.second_order_space)
@@ -168,12 +193,13 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if opts.precision == 0 else "f32 arithmetic, f64 state",
         "data": "synthetic",
         "config": {"workload": args.workload, "mesh": f"{opts.nx}x{opts.ny}x{opts.nz}", "cells": nc,
                    "internal_faces": nif, "boundary_faces": nbf,
                    "physics": "viscous Roe, Green-Gauss, Venkat, 2nd order" if so else "first order",
-                   "cfl": opts.cfl, "last_dt": dt_last, "precision": "fp64"},
+                   "cfl": opts.cfl, "last_dt": dt_last,
+                   "precision": "fp64" if opts.precision == 0 else "fp32 flux/gradient/limiter, fp64 state"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": dom, "algorithmic_bytes_per_launch": alg,
@@ -181,6 +207,8 @@ def main() -> int:
                      "kernels": kernels},
         "cpu_baseline": cpu,
     }
+    if precision_error is not None:
+        line["precision_error"] = precision_error
     print(json.dumps(line), flush=True)
     return 0
 

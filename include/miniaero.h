@@ -46,6 +46,9 @@ enum { MA_SIDE_BOTTOM = 0, MA_SIDE_TOP, MA_SIDE_FRONT, MA_SIDE_BACK, MA_SIDE_RIG
 
 enum { MA_LIMITER_VENKAT = 0, MA_LIMITER_VANALBADA = 1 };
 enum { MA_IC_REFERENCE = 0, MA_IC_SOD = 1, MA_IC_PERTURBED_INFLOW = 2 };
+/* Arithmetic of fluxes, gradients and limiters.  The conserved state, the RK4
+ * accumulation and the residual sums are FP64 in both. */
+enum { MA_PRECISION_FP64 = 0, MA_PRECISION_FP32 = 1 };
 
 /* Options.h:47-57 plus config-mode extensions.  A zeroed extension block
  * reproduces the reference exactly. */
@@ -71,6 +74,7 @@ typedef struct ma_options {
   double cfl;                /* > 0: dt = cfl * min_c vol_c / sum_f(|u.n| + c|n|), per step */
   double perturb;            /* MA_IC_PERTURBED_INFLOW: relative amplitude */
   unsigned long long seed;   /* MA_IC_PERTURBED_INFLOW: generator key */
+  int precision;             /* MA_PRECISION_* */
 } ma_options;
 
 /* One Faces<Device> (Faces.h:42-71), as host arrays. */

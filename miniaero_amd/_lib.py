@@ -64,6 +64,7 @@ class ma_options(ctypes.Structure):
         ("cfl", ctypes.c_double),
         ("perturb", ctypes.c_double),
         ("seed", ctypes.c_ulonglong),
+        ("precision", ctypes.c_int),
     ]
 
 

@@ -96,27 +96,30 @@ struct TileMap {
   const int* __restrict__ cptr;   // ntiles + 1: cells of tile b are [cptr[b], cptr[b+1])
 };
 
-__device__ __forceinline__ void load5(const double* __restrict__ a, int n, int i, double* v) {
+template <class S, class T>
+__device__ __forceinline__ void load5(const S* __restrict__ a, int n, int i, T* v) {
 #pragma unroll
-  for (int k = 0; k < 5; ++k) v[k] = a[(size_t)k * n + i];
+  for (int k = 0; k < 5; ++k) v[k] = T(a[(size_t)k * n + i]);
 }
 
-__device__ __forceinline__ void load3(const double* __restrict__ a, int n, int i, double* v) {
+template <class S, class T>
+__device__ __forceinline__ void load3(const S* __restrict__ a, int n, int i, T* v) {
 #pragma unroll
-  for (int k = 0; k < 3; ++k) v[k] = a[(size_t)k * n + i];
+  for (int k = 0; k < 3; ++k) v[k] = T(a[(size_t)k * n + i]);
 }
 
-__device__ __forceinline__ ma::FaceFrame load_frame(const DevMesh& m, int f) {
-  ma::FaceFrame fr;
+template <class T>
+__device__ __forceinline__ ma::FaceFrameT<T> load_frame(const DevMesh& m, int f) {
+  ma::FaceFrameT<T> fr;
   load3(m.fn, m.nf, f, fr.n);
   const double* p = m.ffr;
   const size_t nf = m.nf;
-  fr.nn = p[f];
+  fr.nn = T(p[f]);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    fr.nu[d] = p[(1 + d) * nf + f];
-    fr.tu[d] = p[(4 + d) * nf + f];
-    fr.bu[d] = p[(7 + d) * nf + f];
+    fr.nu[d] = T(p[(1 + d) * nf + f]);
+    fr.tu[d] = T(p[(4 + d) * nf + f]);
+    fr.bu[d] = T(p[(7 + d) * nf + f]);
   }
   return fr;
 }
@@ -198,9 +201,11 @@ __global__ __launch_bounds__(kBlock) void k_init(DevMesh m, IcParams p, double* 
 
 // Stage state and its primitives: update<Device> with alpha_k
 // (TimeSolverExplicitRK4.h:106-128, 355-357) fused with ComputePrimitives.
+// The stage state is FP64; V is stored in the arithmetic type T.
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_prim(int nc, const double* __restrict__ Un,
                                                  const double* __restrict__ R, double alpha,
-                                                 double* __restrict__ V) {
+                                                 T* __restrict__ V) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   double u[5], v[5];
@@ -208,36 +213,35 @@ __global__ __launch_bounds__(kBlock) void k_prim(int nc, const double* __restric
   for (int k = 0; k < 5; ++k) u[k] = Un[(size_t)k * nc + c] + alpha * R[(size_t)k * nc + c];
   ma::primitives(u, v);
 #pragma unroll
-  for (int k = 0; k < 5; ++k) V[(size_t)k * nc + c] = v[k];
+  for (int k = 0; k < 5; ++k) V[(size_t)k * nc + c] = T(v[k]);
 }
 
 // Green-Gauss gradient (GreenGauss.h:51-270), then, for second order, the
 // stencil min/max (StencilLimiter.h:56-282) and the limiter
 // (StencilLimiter.h:288-500) of one cell, all from the cell's slot list.
-template <bool kSecondOrder, class Limiter>
-__global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const double* __restrict__ V,
-                                                 double* __restrict__ grad,
-                                                 double* __restrict__ phi) {
+template <bool kSecondOrder, class Limiter, class T>
+__global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const T* __restrict__ V,
+                                                 T* __restrict__ grad, T* __restrict__ phi) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= m.nc) return;
   const int nc = m.nc, nf = m.nf;
-  double vc[5];
+  T vc[5];
   load5(V, nc, c, vc);
-  const double vol = m.vol[c];
+  const T vol = T(m.vol[c]);
   int ent[kMaxSlots];
 #pragma unroll
   for (int s = 0; s < kMaxSlots; ++s) ent[s] = m.slot[(size_t)s * nc + c];
 
-  double g[5][3];
+  T g[5][3];
 #pragma unroll
   for (int k = 0; k < 5; ++k)
 #pragma unroll
-    for (int d = 0; d < 3; ++d) g[k][d] = 0.0;
-  double mn[5], mx[5];
+    for (int d = 0; d < 3; ++d) g[k][d] = T(0.0);
+  T mn[5], mx[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
-    mn[k] = 1.0e300;
-    mx[k] = -1.0e300;
+    mn[k] = T(1.0e300);
+    mx[k] = -T(1.0e300);
   }
 
 #pragma unroll
@@ -245,27 +249,27 @@ __global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const double* __rest
     const int e = ent[s];
     if (e < 0) break;
     const int f = e >> 1;
-    double n[3];
+    T n[3];
     load3(m.fn, nf, f, n);
     if (f < m.nif) {
       const int side = e & 1;
       const int nb = side ? m.fl[f] : m.fr[f];
-      double vn[5];
+      T vn[5];
       load5(V, nc, nb, vn);
-      const double* vl = side ? vn : vc;
-      const double* vr = side ? vc : vn;
+      const T* vl = side ? vn : vc;
+      const T* vr = side ? vc : vn;
 #pragma unroll
       for (int d = 0; d < 3; ++d)
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-          const double gg = 0.5 * (vl[k] + vr[k]) * n[d];
+          const T gg = T(0.5) * (vl[k] + vr[k]) * n[d];
           g[k][d] += side ? -gg / vol : gg / vol;
         }
       if (kSecondOrder) {
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-          const double fmin = vr[k] < vl[k] ? vr[k] : vl[k];
-          const double fmax = vr[k] > vl[k] ? vr[k] : vl[k];
+          const T fmin = vr[k] < vl[k] ? vr[k] : vl[k];
+          const T fmax = vr[k] > vl[k] ? vr[k] : vl[k];
           mn[k] = ma::smin(mn[k], fmin);
           mx[k] = ma::smax(mx[k], fmax);
         }
@@ -289,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const double* __rest
   for (int k = 0; k < 5; ++k)
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      double x = 0;
+      T x = 0;
       x += g[k][d];
       g[k][d] = x;
       grad[(size_t)(k * 3 + d) * nc + c] = x;
@@ -298,27 +302,28 @@ __global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const double* __rest
 
   double xcc[3];
   load3(m.xc, nc, c, xcc);
-  double lim[5];
+  T lim[5];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) lim[k] = 1.0;
+  for (int k = 0; k < 5; ++k) lim[k] = T(1.0);
 #pragma unroll
   for (int s = 0; s < kMaxSlots; ++s) {
     const int e = ent[s];
     if (e < 0) break;
     const int f = e >> 1;
-    double disp[3], dist = 0;
+    // displacement from absolute FP64 coordinates, then the arithmetic type
+    T disp[3], dist = 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      disp[d] = m.fx[(size_t)d * nf + f] - xcc[d];
+      disp[d] = T(m.fx[(size_t)d * nf + f] - xcc[d]);
       dist += disp[d] * disp[d];
     }
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-      double du = 0;
+      T du = 0;
 #pragma unroll
       for (int d = 0; d < 3; ++d) du += disp[d] * g[k][d];
-      const double dumax = mx[k] - vc[k];
-      const double dumin = mn[k] - vc[k];
+      const T dumax = mx[k] - vc[k];
+      const T dumin = mn[k] - vc[k];
       lim[k] = ma::smin(lim[k], Limiter::limit(dumax, dumin, du, dist));
     }
   }
@@ -327,17 +332,19 @@ __global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const double* __rest
 }
 
 // Reconstructed (second order) primitive state of cell `c` at face centroid xf:
-// compute_face_flux's extrapolation, Flux.h:109-132.
-__device__ __forceinline__ void reconstruct(const DevMesh& m, const double* __restrict__ grad,
-                                            const double* __restrict__ phi, int c,
-                                            const double* xf, double* v) {
+// compute_face_flux's extrapolation, Flux.h:109-132.  The displacement is
+// taken from the FP64 coordinates, then carried in the arithmetic type T.
+template <class T>
+__device__ __forceinline__ void reconstruct(const DevMesh& m, const T* __restrict__ grad,
+                                            const T* __restrict__ phi, int c, const double* xf,
+                                            T* v) {
   const int nc = m.nc;
-  double dx[3];
+  T dx[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) dx[d] = xf[d] - m.xc[(size_t)d * nc + c];
+  for (int d = 0; d < 3; ++d) dx[d] = T(xf[d] - m.xc[(size_t)d * nc + c]);
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
-    double t = 0;
+    T t = 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) t += dx[d] * grad[(size_t)(k * 3 + d) * nc + c];
     v[k] += t * phi[(size_t)k * nc + c];
@@ -349,17 +356,17 @@ __device__ __forceinline__ void reconstruct(const DevMesh& m, const double* __re
 // the cell limiters, Roe, minus the viscous flux from face-averaged
 // gradients.  The result is a pure function of (f, V, grad, phi), so every
 // workgroup that evaluates face f gets the same bits.
-template <bool kSecondOrder, class Visc>
-__device__ __forceinline__ void interior_flux(const DevMesh& m, const double* __restrict__ V,
-                                              const double* __restrict__ grad,
-                                              const double* __restrict__ phi, int f,
-                                              const Visc& visc, double* F) {
+template <bool kSecondOrder, class Visc, class T>
+__device__ __forceinline__ void interior_flux(const DevMesh& m, const T* __restrict__ V,
+                                              const T* __restrict__ grad,
+                                              const T* __restrict__ phi, int f,
+                                              const Visc& visc, T* F) {
   const int nc = m.nc;
   const int L = m.fl[f], Rr = m.fr[f];
-  double vl[5], vr[5];
+  T vl[5], vr[5];
   load5(V, nc, L, vl);
   load5(V, nc, Rr, vr);
-  const ma::FaceFrame fr = load_frame(m, f);
+  const ma::FaceFrameT<T> fr = load_frame<T>(m, f);
   if (kSecondOrder) {
     double xf[3];
     load3(m.fx, m.nf, f, xf);
@@ -369,13 +376,13 @@ __device__ __forceinline__ void interior_flux(const DevMesh& m, const double* __
   const ma::RoeFlux roe;
   roe.flux_frame(vl, vr, F, fr);
   if (Visc::isViscous) {
-    double vf[5], gf[5][3], fv[5];
+    T vf[5], gf[5][3], fv[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-      vf[k] = 0.5 * (vl[k] + vr[k]);
+      vf[k] = T(0.5) * (vl[k] + vr[k]);
 #pragma unroll
       for (int d = 0; d < 3; ++d)
-        gf[k][d] = 0.5 * (grad[(size_t)(k * 3 + d) * nc + L] + grad[(size_t)(k * 3 + d) * nc + Rr]);
+        gf[k][d] = T(0.5) * (grad[(size_t)(k * 3 + d) * nc + L] + grad[(size_t)(k * 3 + d) * nc + Rr]);
     }
     visc.compute_flux(gf, vf, fr.n, fv);
 #pragma unroll
@@ -388,41 +395,44 @@ __device__ __forceinline__ void interior_flux(const DevMesh& m, const double* __
 // Inflow_BC.h:72-110, -F_inviscid + F_viscous for NoSlip_BC.h:76-156 (whose
 // newtonian viscous part is applied even in inviscid runs,
 // TimeSolverExplicitRK4.h:457-460).  Written to bc[5][nbf].
-__global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const double* __restrict__ V,
-                                               double prandtl, Inflow inflow,
-                                               double* __restrict__ bc) {
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const T* __restrict__ V, double prandtl,
+                                               Inflow inflow, T* __restrict__ bc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int nbf = m.nf - m.nif;
   if (i >= nbf) return;
   const int f = m.nif + i;
   const int c = m.fl[f];
   const int kind = m.fkind[f];
-  double vc[5], vr[5], F[5];
+  T vc[5], vr[5], F[5];
   load5(V, m.nc, c, vc);
-  const ma::FaceFrame fr = load_frame(m, f);
+  const ma::FaceFrameT<T> fr = load_frame<T>(m, f);
   const ma::RoeFlux roe;
-  double out[5];
+  T out[5];
   if (kind == MA_BC_EXTRAPOLATE) {
     roe.flux_frame(vc, vc, F, fr);
 #pragma unroll
     for (int k = 0; k < 5; ++k) out[k] = -F[k];
   } else if (kind == MA_BC_INFLOW) {
-    ma::primitives(inflow.U, vr);
+    double vi[5];
+    ma::primitives(inflow.U, vi);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) vr[k] = T(vi[k]);
     roe.flux_frame(vc, vr, F, fr);
 #pragma unroll
     for (int k = 0; k < 5; ++k) out[k] = -F[k];
   } else {
     // mirror state; area_norm (Tangent_BC.h:87-91) equals |n| of the frame bit for bit
-    const double* n = fr.n;
-    const double an = fr.nn;
-    double ub = 0.0;
+    const T* n = fr.n;
+    const T an = fr.nn;
+    T ub = 0.0;
     ub += vc[1] * n[0] / an;
     ub += vc[2] * n[1] / an;
     ub += vc[3] * n[2] / an;
     vr[0] = vc[0];
-    vr[1] = vc[1] - 2 * ub * n[0] / an;
-    vr[2] = vc[2] - 2 * ub * n[1] / an;
-    vr[3] = vc[3] - 2 * ub * n[2] / an;
+    vr[1] = vc[1] - T(2) * ub * n[0] / an;
+    vr[2] = vc[2] - T(2) * ub * n[1] / an;
+    vr[3] = vc[3] - T(2) * ub * n[2] / an;
     vr[4] = vc[4];
     roe.flux_frame(vc, vr, F, fr);
     if (kind == MA_BC_TANGENT) {
@@ -431,9 +441,9 @@ __global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const double* __restri
     } else {
       // no-slip wall: viscous flux from the one-sided gradient to the wall
       // state (rho, 0, 0, 0, T) over the centroid-to-face distance
-      const double vf[5] = {vc[0], 0.0, 0.0, 0.0, vc[4]};
-      const double inv = m.fwall[f];
-      double gf[5][3], fv[5];
+      const T vf[5] = {vc[0], T(0.0), T(0.0), T(0.0), vc[4]};
+      const T inv = T(m.fwall[f]);
+      T gf[5][3], fv[5];
 #pragma unroll
       for (int k = 0; k < 5; ++k)
 #pragma unroll
@@ -448,7 +458,7 @@ __global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const double* __restri
   for (int k = 0; k < 5; ++k) bc[(size_t)k * nbf + i] = out[k];
 }
 
-// Residual and RK accumulate of one tile of kTile consecutive cells:
+// Residual and RK accumulate of one tile of at most kTile cells:
 // compute_face_flux's atomic scatter (Flux.h:162-170), apply_cell_flux
 // (Flux.h:188-229) and update<Device> with beta_k (TimeSolverExplicitRK4.h:483-485).
 // Phase 1: the workgroup evaluates every internal face touching its cells
@@ -456,19 +466,21 @@ __global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const double* __restri
 // every cell adds its face fluxes from LDS and its boundary contributions in
 // the reference's one-thread order -- internal faces in processing order,
 // then boundary faces by kind (Extrapolate, Tangent, NoSlip, Inflow) and group.
+// The sums, the residual and the accumulator are FP64 whatever T is.
 // Slot entries: (index << 3) | code, code 0 = left side, 1 = right side of an
 // internal face (index = LDS slot), 4 + kind - 1 = boundary face of that kind
 // (index = boundary-face number).
-template <bool kSecondOrder, class Visc>
-__global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const double* __restrict__ V,
-                                                const double* __restrict__ grad,
-                                                const double* __restrict__ phi,
-                                                const double* __restrict__ bc,
+template <bool kSecondOrder, class Visc, class T>
+__global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const T* __restrict__ V,
+                                                const T* __restrict__ grad,
+                                                const T* __restrict__ phi,
+                                                const T* __restrict__ bc,
                                                 const double* Uacc_in,  // may alias Uacc_out
                                                 double* Uacc_out, double* __restrict__ R,
                                                 const double* __restrict__ dt_dev, double beta,
                                                 Visc visc) {
-  extern __shared__ double lds[];  // 5 planes of t.max_faces
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* lds = reinterpret_cast<T*>(lds_raw);  // 5 planes of t.max_faces
   // XCD-aware order: the workgroups an XCD receives (every kXcds-th) take a
   // contiguous run of tiles, so neighbouring tiles share that XCD's L2.
   const int tile = (blockIdx.x % kXcds) * t.per_xcd + blockIdx.x / kXcds;
@@ -477,7 +489,7 @@ __global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const doub
   const int nfl = t.ptr[tile + 1] - f0;
   const int mf = t.max_faces;
   for (int i = threadIdx.x; i < nfl; i += kTile) {
-    double F[5];
+    T F[5];
     interior_flux<kSecondOrder, Visc>(m, V, grad, phi, t.faces[f0 + i], visc, F);
 #pragma unroll
     for (int k = 0; k < 5; ++k) lds[k * mf + i] = F[k];
@@ -500,7 +512,7 @@ __global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const doub
     if (code < 2) {
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
-        const double x = lds[k * mf + j];
+        const double x = double(lds[k * mf + j]);
         acc[k] += code ? x : -x;
       }
     } else {
@@ -517,7 +529,7 @@ __global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const doub
         if ((e & 7) != code) continue;
         const int j = e >> 3;
 #pragma unroll
-        for (int k = 0; k < 5; ++k) acc[k] += bc[(size_t)k * nbf + j];
+        for (int k = 0; k < 5; ++k) acc[k] += double(bc[(size_t)k * nbf + j]);
       }
   }
   const double dt = *dt_dev;
@@ -629,10 +641,13 @@ struct ma_solver {
   DevMesh m{};
   TileMap t{};
   std::vector<int> dev_of;  // caller's cell index -> device cell index
-  double* bc = nullptr;  // boundary-face contributions, 5 planes of nbf
   std::vector<void*> allocs;
   size_t bytes = 0;
-  double *U[2] = {nullptr, nullptr}, *R = nullptr, *V = nullptr, *grad = nullptr, *phi = nullptr;
+  double *U[2] = {nullptr, nullptr}, *R = nullptr;
+  // Arithmetic-type buffers (double, or float for the FP32 form): primitives,
+  // gradients, limiters, boundary contributions.
+  void *V = nullptr, *grad = nullptr, *phi = nullptr, *bc = nullptr;
+  bool fp32 = false;
   double *dt_dev = nullptr, *time_dev = nullptr, *partial = nullptr;
   unsigned long long* counter = nullptr;
   int cur = 0;  // U[cur] holds sol_n
@@ -642,13 +657,16 @@ struct ma_solver {
 
   template <class T>
   ma_status alloc(T** p, size_t n) {
+    return alloc_bytes((void**)p, n * sizeof(T));
+  }
+  ma_status alloc_bytes(void** p, size_t nbytes) {
     *p = nullptr;
-    if (n == 0) n = 1;
-    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (nbytes == 0) nbytes = 8;
+    hipError_t e = hipMalloc(p, nbytes);
     if (e != hipSuccess)
       return ma::fail(MA_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     allocs.push_back(*p);
-    bytes += n * sizeof(T);
+    bytes += nbytes;
     return MA_OK;
   }
 
@@ -1052,7 +1070,7 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
 
   int *dtptr, *dtfaces, *dtslot, *dtcptr;
   if ((st = upload(s, &dtptr, L.tptr)) || (st = upload(s, &dtfaces, L.tfaces)) || (st = upload(s, &dtslot, L.tslot)) ||
-      (st = upload(s, &dtcptr, L.tile_cptr)) || (st = s->alloc(&s->bc, (size_t)5 * std::max<long long>(nbf, 1))))
+      (st = upload(s, &dtcptr, L.tile_cptr)))
     return st;
   s->t.ntiles = L.ntiles;
   s->t.per_xcd = (L.ntiles + kXcds - 1) / kXcds;
@@ -1063,20 +1081,23 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
   s->t.cptr = dtcptr;
 
   const size_t n5 = (size_t)5 * nc;
+  const size_t tsize = s->fp32 ? sizeof(float) : sizeof(double);
   if ((st = s->alloc(&s->U[0], n5)) || (st = s->alloc(&s->U[1], n5)) || (st = s->alloc(&s->R, n5)) ||
-      (st = s->alloc(&s->V, n5)) || (st = s->alloc(&s->grad, 3 * n5)) || (st = s->alloc(&s->phi, n5)) ||
+      (st = s->alloc_bytes(&s->V, n5 * tsize)) || (st = s->alloc_bytes(&s->grad, 3 * n5 * tsize)) ||
+      (st = s->alloc_bytes(&s->phi, n5 * tsize)) ||
+      (st = s->alloc_bytes(&s->bc, (size_t)5 * std::max<long long>(nbf, 1) * tsize)) ||
       (st = s->alloc(&s->dt_dev, 1)) || (st = s->alloc(&s->time_dev, 1)) ||
       (st = s->alloc(&s->partial, (size_t)nblocks(nc))) || (st = s->alloc(&s->counter, 1)))
     return st;
   MA_HIP(hipMemset(s->R, 0, n5 * sizeof(double)));
-  MA_HIP(hipMemset(s->grad, 0, 3 * n5 * sizeof(double)));
-  MA_HIP(hipMemset(s->phi, 0, n5 * sizeof(double)));
+  MA_HIP(hipMemset(s->grad, 0, 3 * n5 * tsize));
+  MA_HIP(hipMemset(s->phi, 0, n5 * tsize));
   MA_HIP(hipMemcpy(s->dt_dev, &s->opt.dt, sizeof(double), hipMemcpyHostToDevice));
   MA_HIP(hipMemset(s->time_dev, 0, sizeof(double)));
   return MA_OK;
 }
 
-template <bool SO, class Visc, class Lim>
+template <bool SO, class Visc, class Lim, class T>
 void run_stage(ma_solver* s, int irk) {
   // TimeSolverExplicitRK4.h:187-195
   constexpr double alpha[4] = {0.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};
@@ -1086,11 +1107,13 @@ void run_stage(ma_solver* s, int irk) {
   double* Un = s->U[s->cur];
   double* Uacc = s->U[1 - s->cur];
   s->launch("k_prim", [&] {
-    hipLaunchKernelGGL(k_prim, dim3(nb), dim3(kBlock), 0, s->stream, m.nc, Un, s->R, alpha[irk], s->V);
+    hipLaunchKernelGGL(k_prim<T>, dim3(nb), dim3(kBlock), 0, s->stream, m.nc, Un, s->R, alpha[irk],
+                       static_cast<T*>(s->V));
   });
   if (SO || Visc::isViscous) {
     s->launch(SO ? "k_grad_limiter" : "k_grad", [&] {
-      hipLaunchKernelGGL((k_grad<SO, Lim>), dim3(nb), dim3(kBlock), 0, s->stream, m, s->V, s->grad, s->phi);
+      hipLaunchKernelGGL((k_grad<SO, Lim, T>), dim3(nb), dim3(kBlock), 0, s->stream, m,
+                         static_cast<const T*>(s->V), static_cast<T*>(s->grad), static_cast<T*>(s->phi));
     });
   }
   Visc visc{s->opt.prandtl > 0.0 ? s->opt.prandtl : 0.71};
@@ -1099,18 +1122,20 @@ void run_stage(ma_solver* s, int irk) {
   const int nbf = m.nf - m.nif;
   if (nbf > 0) {
     s->launch("k_bc", [&] {
-      hipLaunchKernelGGL(k_bc, dim3(nblocks(nbf)), dim3(kBlock), 0, s->stream, m, s->V, visc.prandtl,
-                         s->inflow, s->bc);
+      hipLaunchKernelGGL(k_bc<T>, dim3(nblocks(nbf)), dim3(kBlock), 0, s->stream, m,
+                         static_cast<const T*>(s->V), visc.prandtl, s->inflow, static_cast<T*>(s->bc));
     });
   }
-  const size_t lds = (size_t)s->t.max_faces * 5 * sizeof(double);
+  const size_t lds = (size_t)s->t.max_faces * 5 * sizeof(T);
   s->launch("k_flux", [&] {
-    hipLaunchKernelGGL((k_flux<SO, Visc>), dim3(s->t.per_xcd * kXcds), dim3(kTile), lds, s->stream, m, s->t,
-                       s->V, s->grad, s->phi, s->bc, acc_in, Uacc, s->R, s->dt_dev, beta[irk], visc);
+    hipLaunchKernelGGL((k_flux<SO, Visc, T>), dim3(s->t.per_xcd * kXcds), dim3(kTile), lds, s->stream, m,
+                       s->t, static_cast<const T*>(s->V), static_cast<const T*>(s->grad),
+                       static_cast<const T*>(s->phi), static_cast<const T*>(s->bc), acc_in, Uacc, s->R,
+                       s->dt_dev, beta[irk], visc);
   });
 }
 
-template <bool SO, class Visc, class Lim>
+template <bool SO, class Visc, class Lim, class T>
 void run_steps(ma_solver* s, int nsteps) {
   for (int it = 0; it < nsteps; ++it) {
     if (s->opt.cfl > 0.0) {
@@ -1126,31 +1151,40 @@ void run_steps(ma_solver* s, int nsteps) {
       s->time += s->opt.dt;
       s->last_dt = s->opt.dt;
     }
-    for (int irk = 0; irk < 4; ++irk) run_stage<SO, Visc, Lim>(s, irk);
+    for (int irk = 0; irk < 4; ++irk) run_stage<SO, Visc, Lim, T>(s, irk);
     s->cur = 1 - s->cur;  // U_acc after the last stage is the new sol_n
   }
 }
 
-template <bool SO, class Visc>
+template <bool SO, class Visc, class T>
 void dispatch_lim(ma_solver* s, int n) {
   if (s->opt.limiter == MA_LIMITER_VANALBADA)
-    run_steps<SO, Visc, ma::VanAlbadaLimiter>(s, n);
+    run_steps<SO, Visc, ma::VanAlbadaLimiter, T>(s, n);
   else
-    run_steps<SO, Visc, ma::VenkatLimiter>(s, n);
+    run_steps<SO, Visc, ma::VenkatLimiter, T>(s, n);
+}
+
+template <class T>
+void dispatch_t(ma_solver* s, int n) {
+  const bool so = s->opt.second_order_space != 0, visc = s->opt.viscous != 0;
+  if (so && visc) dispatch_lim<true, ma::NewtonianViscousFlux, T>(s, n);
+  else if (so) dispatch_lim<true, ma::NoViscousFlux, T>(s, n);
+  else if (visc) dispatch_lim<false, ma::NewtonianViscousFlux, T>(s, n);
+  else dispatch_lim<false, ma::NoViscousFlux, T>(s, n);
 }
 
 void dispatch(ma_solver* s, int n) {
-  const bool so = s->opt.second_order_space != 0, visc = s->opt.viscous != 0;
-  if (so && visc) dispatch_lim<true, ma::NewtonianViscousFlux>(s, n);
-  else if (so) dispatch_lim<true, ma::NoViscousFlux>(s, n);
-  else if (visc) dispatch_lim<false, ma::NewtonianViscousFlux>(s, n);
-  else dispatch_lim<false, ma::NoViscousFlux>(s, n);
+  if (s->fp32)
+    dispatch_t<float>(s, n);
+  else
+    dispatch_t<double>(s, n);
 }
 
-ma_status download_aos(ma_solver* s, const double* dsrc, int ncomp, double* host) {
+template <class S>
+ma_status download_aos(ma_solver* s, const S* dsrc, int ncomp, double* host) {
   const int nc = s->m.nc;
-  std::vector<double> tmp((size_t)ncomp * nc);
-  MA_HIP(hipMemcpyAsync(tmp.data(), dsrc, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  std::vector<S> tmp((size_t)ncomp * nc);
+  MA_HIP(hipMemcpyAsync(tmp.data(), dsrc, tmp.size() * sizeof(S), hipMemcpyDeviceToHost, s->stream));
   MA_HIP(hipStreamSynchronize(s->stream));
   for (int c = 0; c < nc; ++c) {
     const int d = s->dev_of[c];
@@ -1200,12 +1234,15 @@ extern "C" ma_status ma_solver_create(const ma_mesh_data* md, const ma_options* 
   if (opt->cfl <= 0.0 && !(opt->dt > 0.0)) return ma::fail(MA_ERR_INVALID, "dt must be positive");
   if (opt->limiter != MA_LIMITER_VENKAT && opt->limiter != MA_LIMITER_VANALBADA)
     return ma::fail(MA_ERR_INVALID, "unknown limiter");
+  if (opt->precision != MA_PRECISION_FP64 && opt->precision != MA_PRECISION_FP32)
+    return ma::fail(MA_ERR_INVALID, "unknown precision");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return ma::fail(MA_ERR_DEVICE, "no HIP device available");
   ma_solver* s = new (std::nothrow) ma_solver();
   if (!s) return ma::fail(MA_ERR_NOMEM, "ma_solver_create: out of host memory");
   s->opt = *opt;
+  s->fp32 = opt->precision == MA_PRECISION_FP32;
   // TimeSolverExplicitRK4.h:218-223
   const double ref_inflow[5] = {0.5805, 503.96, 0.0, 0.0, 343750.0};
   for (int k = 0; k < 5; ++k) s->inflow.U[k] = opt->use_inflow ? opt->inflow[k] : ref_inflow[k];
@@ -1292,8 +1329,13 @@ extern "C" ma_status ma_solver_get_residual(ma_solver* s, double* R) {
 extern "C" ma_status ma_solver_get_gradients(ma_solver* s, double* grad, double* lim) {
   if (!s) return ma::fail(MA_ERR_INVALID, "null solver");
   ma_status st = MA_OK;
-  if (grad && (st = download_aos(s, s->grad, 15, grad))) return st;
-  if (lim && (st = download_aos(s, s->phi, 5, lim))) return st;
+  if (s->fp32) {
+    if (grad && (st = download_aos(s, static_cast<const float*>(s->grad), 15, grad))) return st;
+    if (lim && (st = download_aos(s, static_cast<const float*>(s->phi), 5, lim))) return st;
+  } else {
+    if (grad && (st = download_aos(s, static_cast<const double*>(s->grad), 15, grad))) return st;
+    if (lim && (st = download_aos(s, static_cast<const double*>(s->phi), 5, lim))) return st;
+  }
   return MA_OK;
 }
 

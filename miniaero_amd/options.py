@@ -38,6 +38,7 @@ class Options:
     cfl: float = 0.0
     perturb: float = 0.0
     seed: int = 0
+    precision: int = 0        # 0 FP64, 1 FP32 flux/gradient/limiter arithmetic
 
     def read_options_file(self, path: str = "miniaero.inp") -> "Options":
         """Options::read_options_file (Options.h:73-101): nine whitespace
