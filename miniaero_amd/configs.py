@@ -58,8 +58,10 @@ def flat_plate_c2(**kw) -> Options:
 
 def flat_plate_c3(**kw) -> Options:
     """The C2 physics on 512x256x64, started from the freestream with a seeded
-    +-1% perturbation of density and velocity, 500 steps."""
-    return flat_plate_c2(nx=512, ny=256, nz=64, ntimesteps=500, ic_kind=2, perturb=0.01, seed=3).replace(**kw)
+    +-1% perturbation of density and velocity, 500 steps, flux/gradient/limiter
+    arithmetic in FP32 with FP64 state and sums."""
+    return flat_plate_c2(nx=512, ny=256, nz=64, ntimesteps=500, ic_kind=2, perturb=0.01, seed=3,
+                         precision=1).replace(**kw)
 
 
 WORKLOADS = {"sod_c1": sod_c1, "flat_plate_c2": flat_plate_c2, "flat_plate_c3": flat_plate_c3}
