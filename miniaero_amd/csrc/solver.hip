@@ -221,8 +221,11 @@ __global__ __launch_bounds__(kBlock) void k_prim(int nc, const double* __restric
 // (StencilLimiter.h:288-500) of one cell, all from the cell's slot list.
 template <bool kSecondOrder, class Limiter, class T>
 __global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const T* __restrict__ V,
-                                                 T* __restrict__ grad, T* __restrict__ phi) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+                                                 T* __restrict__ grad, T* __restrict__ phi,
+                                                 int per_xcd) {
+  // XCD-aware: each XCD takes a contiguous run of blocks (see k_flux)
+  const int b = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  const int c = b * kBlock + threadIdx.x;
   if (c >= m.nc) return;
   const int nc = m.nc, nf = m.nf;
   T vc[5];
@@ -543,9 +546,13 @@ __global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const T* _
   }
 }
 
-// Stable time step for the CFL extension: per cell vol / sum_f (|u.n| + c|n|)
-// over the slot list, reduced to the minimum per block.
+// Stable time step for the CFL extension, per cell over its slot list:
+//   lc = sum_f (|u.n_f| + c |n_f|),  lv = sum_f |n_f|^2,
+//   dt_c = vol / (lc + 4 kappa lv / vol),  kappa = max(4/3, gamma/Pr) mu / rho
+// (kappa = 0 for inviscid runs: convective plus viscous spectral radius),
+// reduced to the minimum per block.
 __global__ __launch_bounds__(kBlock) void k_dt_partial(DevMesh m, const double* __restrict__ Un,
+                                                       int viscous, double prandtl,
                                                        double* __restrict__ partial) {
   __shared__ double red[kBlock / 64];
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -556,17 +563,27 @@ __global__ __launch_bounds__(kBlock) void k_dt_partial(DevMesh m, const double* 
     load5(Un, nc, c, u);
     ma::primitives(u, v);
     const double snd = sqrt(1.4 * (v[0] * ma::kRgas * v[4]) / v[0]);
-    double s = 0.0;
+    double lc = 0.0, lv = 0.0;
     for (int q = 0; q < kMaxSlots; ++q) {
       const int e = m.slot[(size_t)q * nc + c];
       if (e < 0) break;
       const int f = e >> 1;
       double n[3];
       load3(m.fn, nf, f, n);
-      const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-      s = s + (fabs(v[1] * n[0] + v[2] * n[1] + v[3] * n[2]) + snd * nn);
+      const double n2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+      const double nn = sqrt(n2);
+      lc = lc + (fabs(v[1] * n[0] + v[2] * n[1] + v[3] * n[2]) + snd * nn);
+      lv = lv + n2;
     }
-    dtc = m.vol[c] / s;
+    const double vol = m.vol[c];
+    double lam = lc;
+    if (viscous) {
+      const double kfac = ma::smax(4.0 / 3.0, 1.4 / prandtl);
+      const double mu = 1.458e-6 * v[4] * sqrt(v[4]) / (v[4] + 110.4);
+      const double kappa = kfac * mu / v[0];
+      lam = lam + 4.0 * kappa * lv / vol;
+    }
+    dtc = vol / lam;
   }
   for (int off = 32; off > 0; off >>= 1) dtc = ma::smin(dtc, __shfl_down(dtc, off, 64));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1112,8 +1129,10 @@ void run_stage(ma_solver* s, int irk) {
   });
   if (SO || Visc::isViscous) {
     s->launch(SO ? "k_grad_limiter" : "k_grad", [&] {
-      hipLaunchKernelGGL((k_grad<SO, Lim, T>), dim3(nb), dim3(kBlock), 0, s->stream, m,
-                         static_cast<const T*>(s->V), static_cast<T*>(s->grad), static_cast<T*>(s->phi));
+      const int per = (nb + kXcds - 1) / kXcds;
+      hipLaunchKernelGGL((k_grad<SO, Lim, T>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, m,
+                         static_cast<const T*>(s->V), static_cast<T*>(s->grad), static_cast<T*>(s->phi),
+                         per);
     });
   }
   Visc visc{s->opt.prandtl > 0.0 ? s->opt.prandtl : 0.71};
@@ -1141,7 +1160,8 @@ void run_steps(ma_solver* s, int nsteps) {
     if (s->opt.cfl > 0.0) {
       const int nb = nblocks(s->m.nc);
       s->launch("k_dt_partial", [&] {
-        hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(kBlock), 0, s->stream, s->m, s->U[s->cur], s->partial);
+        hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(kBlock), 0, s->stream, s->m, s->U[s->cur],
+                           s->opt.viscous, s->opt.prandtl > 0.0 ? s->opt.prandtl : 0.71, s->partial);
       });
       s->launch("k_dt_final", [&] {
         hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kBlock), 0, s->stream, s->partial, nb, s->opt.cfl,

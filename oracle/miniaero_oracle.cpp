@@ -905,18 +905,23 @@ void bc_flux(orc_solver* s, const Faces& G, int kind) {
   }
 }
 
-// Stable dt for the CFL extension: per cell vol / sum over faces of (|u.n| + c|n|),
-// faces in the Green-Gauss order; minimum over cells.
+// Stable dt for the CFL extension, faces in the Green-Gauss order:
+//   lc = sum_f (|u.n_f| + c |n_f|),  lv = sum_f |n_f|^2,
+//   dt_c = vol / (lc + 4 kappa lv / vol),  kappa = max(4/3, gamma/Pr) mu / rho
+// (kappa = 0 for inviscid runs; the convective plus viscous spectral radius
+// of Blazek, Computational Fluid Dynamics, sec. 6.1.4), dt = cfl * min_c dt_c.
 double stable_dt(orc_solver* s) {
   const orc_mesh* m = s->m;
-  std::vector<double> acc((size_t)s->nc, 0.0);
+  std::vector<double> lc((size_t)s->nc, 0.0), lv((size_t)s->nc, 0.0);
   std::vector<double> V((size_t)s->nc * 5);
   for (int c = 0; c < s->nc; ++c) prim_of(s->sol_n, c, &V[(size_t)c * 5]);
   auto add = [&](int c, const double* n) {
     const double* p = &V[(size_t)c * 5];
     const double snd = std::sqrt(1.4 * (p[0] * kRgas * p[4]) / p[0]);
-    const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-    acc[c] = acc[c] + (std::fabs(p[1] * n[0] + p[2] * n[1] + p[3] * n[2]) + snd * nn);
+    const double n2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+    const double nn = std::sqrt(n2);
+    lc[c] = lc[c] + (std::fabs(p[1] * n[0] + p[2] * n[1] + p[3] * n[2]) + snd * nn);
+    lv[c] = lv[c] + n2;
   };
   const Faces& F = m->internal;
   for (int ii = 0; ii < F.n; ++ii) {
@@ -926,9 +931,19 @@ double stable_dt(orc_solver* s) {
   }
   for (const Faces& G : m->groups)
     for (int i = 0; i < G.n; ++i) add(G.conn[2 * i], &G.a[3 * i]);
+  const double Pr = s->o.prandtl > 0.0 ? s->o.prandtl : 0.71;
+  const double kfac = std::max(4.0 / 3.0, 1.4 / Pr);
   double dtmin = 1e300;
   for (int c = 0; c < s->nc; ++c) {
-    const double d = m->cell_vol[c] / acc[c];
+    const double* p = &V[(size_t)c * 5];
+    const double vol = m->cell_vol[c];
+    double lam = lc[c];
+    if (s->o.viscous) {
+      const double mu = 1.458e-6 * p[4] * std::sqrt(p[4]) / (p[4] + 110.4);
+      const double kappa = kfac * mu / p[0];
+      lam = lam + 4.0 * kappa * lv[c] / vol;
+    }
+    const double d = vol / lam;
     dtmin = d < dtmin ? d : dtmin;
   }
   return s->o.cfl * dtmin;
