@@ -133,7 +133,9 @@ void ma_mesh_destroy(ma_mesh* mesh);
  * arrays and builds the device layout ma_solver_create would use.  stats[0..2]
  * = lattice dimensions found from the face connectivity (0 when the mesh is
  * not a lattice), [3] = flux tiles, [4] = most internal faces in one tile,
- * [5] = tile face entries in total.  No reference counterpart. */
+ * [5] = tile face entries in total, [6] = 1 when every face's tangent and
+ * binormal follow from its area vector as in Face.C (the kernels then rebuild
+ * the face frame instead of reading it).  No reference counterpart. */
 ma_status ma_mesh_plan(const ma_mesh_data* mesh, long long* stats);
 
 /* ---- solver (replaces TimeSolverExplicitRK4<Device>, TimeSolverExplicitRK4.h:160-539) ---- */

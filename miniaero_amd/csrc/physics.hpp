@@ -83,6 +83,44 @@ MA_HD FaceFrame make_frame(const double* fn, const double* ft, const double* fb)
   return f;
 }
 
+// The tangent and binormal the reference attaches to a face are functions of
+// its area vector a alone (Face.C:80-96): i1 = first index of the largest
+// |a_i|, t[i1+1] = 0, t[i1] = a[i1+2]/den, t[i1+2] = -a[i1]/den with
+// den = sqrt(a[i1]^2 + a[i1+2]^2) (indices mod 3), b = a x t.  When a mesh's
+// t and b are exactly these, the frame can be rebuilt from a on the fly with
+// the same operations, which is cheaper than reading it.
+MA_HD void tangent_binormal(const double* a, double* t, double* b) {
+  const double m0 = fabs(a[0]), m1 = fabs(a[1]), m2 = fabs(a[2]);
+  int i1 = 0;
+  if (m1 > m0) i1 = 1;
+  if (m2 > (i1 == 0 ? m0 : m1)) i1 = 2;
+  if (i1 == 0) {
+    const double den = sqrt(a[0] * a[0] + a[2] * a[2]);
+    t[0] = a[2] / den;
+    t[1] = 0.0;
+    t[2] = -a[0] / den;
+  } else if (i1 == 1) {
+    const double den = sqrt(a[1] * a[1] + a[0] * a[0]);
+    t[0] = -a[1] / den;
+    t[1] = a[0] / den;
+    t[2] = 0.0;
+  } else {
+    const double den = sqrt(a[2] * a[2] + a[1] * a[1]);
+    t[0] = 0.0;
+    t[1] = -a[2] / den;
+    t[2] = a[1] / den;
+  }
+  b[0] = a[1] * t[2] - t[1] * a[2];
+  b[1] = -a[0] * t[2] + t[0] * a[2];
+  b[2] = a[0] * t[1] - t[0] * a[1];
+}
+
+MA_HD FaceFrame frame_from_area(const double* a) {
+  double t[3], b[3];
+  tangent_binormal(a, t, b);
+  return make_frame(a, t, b);
+}
+
 struct RoeFlux {
   // Roe_Flux.h:50-54 signature: builds the face frame, then the flux.
   MA_HD void compute_flux(const double* pl, const double* pr, double* flux, const double* fn,

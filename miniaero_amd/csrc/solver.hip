@@ -71,7 +71,8 @@ struct DevMesh {
   const int* __restrict__ org;     // device cell -> caller's cell index
   const int* __restrict__ slot;    // kMaxSlots planes of nc; entry = face*2 + side, -1 empty
   const double* __restrict__ fn;   // 3 planes of nf: area vector
-  const double* __restrict__ ffr;  // 10 planes of nf: |n|, unit n, unit t, unit b
+  const double* __restrict__ ffr;  // 10 planes of nf: |n|, unit n, unit t, unit b (or null)
+  int derive_frame;                // 1: rebuild the frame from fn (ffr unused)
   const double* __restrict__ fx;
   const int* __restrict__ fl;
   const int* __restrict__ fr;
@@ -111,6 +112,20 @@ __device__ __forceinline__ void load3(const S* __restrict__ a, int n, int i, T* 
 template <class T>
 __device__ __forceinline__ ma::FaceFrameT<T> load_frame(const DevMesh& m, int f) {
   ma::FaceFrameT<T> fr;
+  if (m.derive_frame) {
+    double a[3];
+    load3(m.fn, m.nf, f, a);
+    const ma::FaceFrame d = ma::frame_from_area(a);
+    fr.nn = T(d.nn);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      fr.n[k] = T(d.n[k]);
+      fr.nu[k] = T(d.nu[k]);
+      fr.tu[k] = T(d.tu[k]);
+      fr.bu[k] = T(d.bu[k]);
+    }
+    return fr;
+  }
   load3(m.fn, m.nf, f, fr.n);
   const double* p = m.ffr;
   const size_t nf = m.nf;
@@ -840,6 +855,7 @@ void cell_order(const ma_mesh_data* md, std::vector<int>& dev_of, std::vector<in
 struct Layout {
   int nc = 0, nif = 0, nf = 0, nbf = 0, ntiles = 0, max_faces = 1;
   int dims[3] = {0, 0, 0};  // mesh lattice, zeros when there is none
+  bool derive_frame = true;  // every face's t and b follow from its area vector
   std::vector<int> dev_of, org_of, tile_cptr, tptr, tfaces, slot, tslot, fl, fr;
   std::vector<double> fn, ffr, fx, fwall, vol, xc;
   std::vector<signed char> fkind;
@@ -909,6 +925,14 @@ ma_status plan_layout(const ma_mesh_data* md, Layout& L) {
   fr.assign(nf, 0);
   fkind.assign(nf, 0);
   auto put = [&](int f, const ma_faces& F, int i) {
+    const double* a = &F.face_normal[3 * i];
+    double t[3], b[3];
+    ma::tangent_binormal(a, t, b);
+    for (int d = 0; d < 3; ++d)
+      if (t[d] != F.face_tangent[3 * i + d] || b[d] != F.face_binormal[3 * i + d] ||
+          std::signbit(t[d]) != std::signbit(F.face_tangent[3 * i + d]) ||
+          std::signbit(b[d]) != std::signbit(F.face_binormal[3 * i + d]))
+        L.derive_frame = false;
     const ma::FaceFrame fr = ma::make_frame(&F.face_normal[3 * i], &F.face_tangent[3 * i],
                                             &F.face_binormal[3 * i]);
     ffr[f] = fr.nn;
@@ -1065,11 +1089,12 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
   m.nc = nc;
   m.nif = nif;
   m.nf = nf;
-  double *dvol, *dxc, *dfn, *dffr, *dfx, *dfw;
+  double *dvol, *dxc, *dfn, *dffr = nullptr, *dfx, *dfw;
   int *dslot, *dfl, *dfr, *dorg;
   signed char* dkind;
   if ((st = upload(s, &dvol, L.vol)) || (st = upload(s, &dxc, L.xc)) || (st = upload(s, &dslot, L.slot)) ||
-      (st = upload(s, &dfn, L.fn)) || (st = upload(s, &dffr, L.ffr)) || (st = upload(s, &dorg, L.org_of)) ||
+      (st = upload(s, &dfn, L.fn)) || (!L.derive_frame && (st = upload(s, &dffr, L.ffr))) ||
+      (st = upload(s, &dorg, L.org_of)) ||
       (st = upload(s, &dfx, L.fx)) || (st = upload(s, &dfl, L.fl)) || (st = upload(s, &dfr, L.fr)) ||
       (st = upload(s, &dkind, L.fkind)) || (st = upload(s, &dfw, L.fwall)))
     return st;
@@ -1078,7 +1103,8 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
   m.slot = dslot;
   m.org = dorg;
   m.fn = dfn;
-  m.ffr = dffr;
+  m.ffr = L.derive_frame ? nullptr : dffr;
+  m.derive_frame = L.derive_frame ? 1 : 0;
   m.fx = dfx;
   m.fl = dfl;
   m.fr = dfr;
@@ -1239,6 +1265,7 @@ extern "C" ma_status ma_mesh_plan(const ma_mesh_data* md, long long* stats) {
   stats[3] = L.ntiles;
   stats[4] = L.max_faces;
   stats[5] = L.tptr.empty() ? 0 : L.tptr.back();
+  stats[6] = L.derive_frame ? 1 : 0;
   return MA_OK;
 }
 

@@ -68,10 +68,10 @@ class MeshData:
     def plan(self) -> dict:
         """Host-only pre-flight check (ma_mesh_plan): validates the arrays and
         reports the device layout the solver would build."""
-        st = (ctypes.c_longlong * 6)()
+        st = (ctypes.c_longlong * 7)()
         _lib.check(_lib.load().ma_mesh_plan(ctypes.byref(self.to_c()), st))
         return dict(lattice=(st[0], st[1], st[2]), tiles=st[3], max_tile_faces=st[4],
-                    tile_face_entries=st[5])
+                    tile_face_entries=st[5], derived_frames=bool(st[6]))
 
     def to_c(self) -> _lib.ma_mesh_data:
         m = _lib.ma_mesh_data()

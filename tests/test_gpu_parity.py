@@ -332,3 +332,27 @@ def test_fp32_arithmetic_form_error_vs_fp64(name):
          f"raw per component {['%.2e' % x for x in raw]}")
     assert whole <= 1e-5 and max(per) <= 1e-5
     assert not np.array_equal(runs[1], runs[0])
+
+
+def test_stored_face_frames_match_derived_frames():
+    """A mesh whose tangents and binormals are scaled by 2 has the same unit
+    vectors (scaling by 2 is exact) but no longer follows the reference's
+    construction, so the solver reads stored frames instead of rebuilding
+    them: both paths must give the same bits."""
+    kw = _case("Ramp_Serial")
+    md = M.Parallel3DMesh(kw["nx"], kw["ny"], kw["nz"], kw["lx"], kw["ly"], kw["lz"],
+                          kw["problem_type"], kw["angle"]).fillMeshData()
+    def scaled(F):
+        return M.Faces(F.coordinates, F.face_normal, 2.0 * F.face_tangent, 2.0 * F.face_binormal,
+                       F.face_cell_conn, F.cell_flux_index, F.permute_vector)
+    md2 = M.MeshData(md.num_owned_cells, md.cell_coordinates, md.cell_volumes,
+                     scaled(md.internal_faces), [(n, scaled(F)) for n, F in md.boundary_faces])
+    assert md.plan()["derived_frames"] and not md2.plan()["derived_frames"]
+    out = []
+    for mesh in (md, md2):
+        s = M.TimeSolverExplicitRK4(mesh, M.Options(**kw))
+        s.initialize()
+        s.step(20)
+        out.append(s.state())
+        s.close()
+    assert np.array_equal(out[0], out[1])

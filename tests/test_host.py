@@ -165,3 +165,15 @@ def test_layout_plan_rejects_bad_meshes():
         M.MeshData(md.num_owned_cells, md.cell_coordinates, md.cell_volumes, F, md.boundary_faces,
                    num_ghosts=4).plan()
     assert e.value.status == _lib.MA_ERR_UNSUPPORTED
+
+
+def test_layout_plan_detects_derivable_face_frames():
+    md = M.Parallel3DMesh(6, 5, 4, 2.0, 2.0, 1.0, 2, 30.0).fillMeshData()
+    assert md.plan()["derived_frames"]
+    F = md.internal_faces
+    t = F.face_tangent.copy()
+    t[3, 0] = np.nextafter(t[3, 0], 9.0)
+    bad = M.MeshData(md.num_owned_cells, md.cell_coordinates, md.cell_volumes,
+                     M.Faces(F.coordinates, F.face_normal, t, F.face_binormal, F.face_cell_conn),
+                     md.boundary_faces)
+    assert not bad.plan()["derived_frames"]
