@@ -29,48 +29,82 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(kernel: str, nc: int, nif: int, nbf: int, second_order: bool) -> float:
+def algorithmic_bytes(kernel: str, nc: int, nif: int, nbf: int, second_order: bool,
+                      t: int = 8, g: int = 8) -> float:
     """Compulsory HBM bytes of one launch: every array the kernel needs read
-    once, every output written once (FP64 values, int32 indices)."""
+    once, every output written once.  t = bytes of a primitive, gradient or
+    limiter value (8 FP64, 4 in the FP32-arithmetic form), g = bytes of a
+    geometry value (cell centroid and volume, face area/tangent/binormal,
+    face centroid; also 4 in the FP32-arithmetic form); the state U, the
+    RK accumulator and the residual are always FP64 (40 B per cell each);
+    connectivity is 8 B per face."""
     F = nif + nbf
     if kernel == "k_flux":
-        # V 40, grad 120, phi 40, x_c 24, vol 8, U_acc 40 read; U_acc 40, R 40 written per
-        # cell; left/right 8, n/t/b 72, x_f 24 per face.
+        # V, grad, phi, x_c, vol, U_acc read; U_acc, R written per cell;
+        # left/right, area/tangent/binormal, x_f per face
         if second_order:
-            return 352.0 * nc + 104.0 * F
-        return 168.0 * nc + 80.0 * F       # V, vol, U_acc in; U_acc, R out; conn, n/t/b
+            return (25 * t + 4 * g + 120.0) * nc + (8 + 12 * g) * F
+        return (5 * t + g + 120.0) * nc + (8 + 9 * g) * F
     if kernel in ("k_grad", "k_grad_limiter"):
-        # V 40, vol 8, x_c 24 read; grad 120 (+ phi 40) written; conn 8, n 24, x_f 24 per face
-        return (232.0 if kernel == "k_grad_limiter" else 192.0) * nc + 56.0 * F
+        # V, vol, x_c read; grad (+ phi) written; left/right, area, x_f per face
+        vals = 25 if kernel == "k_grad_limiter" else 20
+        return (vals * t + 4 * g) * nc + (8 + 6 * g) * F
     if kernel == "k_prim":
-        return 120.0 * nc                  # U_n, R read; V written
+        return (80.0 + 5 * t) * nc          # U_n, R read; V written
     if kernel == "k_dt_partial":
         return 48.0 * nc + 24.0 * F
     return 0.0
 
 
-def cpu_baseline(opts) -> dict:
-    """Times the CPU oracle on a bounded sample: the same physics and options
-    on a 128x64x32 mesh (1/8 of the cells), 2 RK4 steps."""
+def _oracle_rate(cases, steps: int):
+    """Cell-stage updates/s of the CPU oracle over the given options, run one
+    after another for `steps` RK4 steps each."""
     import dataclasses
 
     from oracle import oracle as O
 
-    kw = {f.name: getattr(opts, f.name) for f in dataclasses.fields(opts)
-          if f.name not in ("output_results", "output_frequency")}
-    kw.update(nx=max(1, opts.nx // 2), ny=max(1, opts.ny // 2), nz=max(1, opts.nz // 2))
-    steps = 2
-    m = O.OracleMesh(O.make_options(**kw))
-    s = O.OracleSolver(m)
-    s.init()
-    t0 = time.perf_counter()
-    s.step(steps)
-    dt = time.perf_counter() - t0
-    return {"value": m.ncells * 4 * steps / dt, "unit": "cell-stage updates/s", "cores": 1,
-            "kind": "port",
+    cells, secs = 0, 0.0
+    for opts in cases:
+        kw = {f.name: getattr(opts, f.name) for f in dataclasses.fields(opts)
+              if f.name not in ("output_results", "output_frequency")}
+        m = O.OracleMesh(O.make_options(**kw))
+        s = O.OracleSolver(m)
+        s.init()
+        t0 = time.perf_counter()
+        s.step(steps)
+        secs += time.perf_counter() - t0
+        cells += m.ncells
+    return cells * 4 * steps / secs, cells, secs
+
+
+def cpu_baseline(opts) -> dict:
+    """Times the CPU oracle on a bounded sample: the same physics and options
+    on a mesh with half the cells in each direction (1/8 of the cells), 5
+    RK4 steps."""
+    steps = 5
+    sample = opts.replace(nx=max(1, opts.nx // 2), ny=max(1, opts.ny // 2), nz=max(1, opts.nz // 2))
+    rate, cells, secs = _oracle_rate([sample], steps)
+    return {"value": rate, "unit": "cell-stage updates/s", "cores": 1, "kind": "port",
             "sample": f"oracle (single-thread C++ restatement of the reference's serial path), "
-                      f"{kw['nx']}x{kw['ny']}x{kw['nz']} = {m.ncells} cells, {steps} RK4 steps, "
-                      f"same physics/options, {dt:.1f} s"}
+                      f"{sample.nx}x{sample.ny}x{sample.nz} = {cells} cells, {steps} RK4 steps, "
+                      f"same physics/options, {secs:.1f} s"}
+
+
+def cpu_baseline_batch(cases, min_cells: int = 600_000) -> dict:
+    """The CPU oracle over the batch's first cases, one after another, until
+    they hold min_cells cells, 3 RK4 steps each."""
+    steps = 3
+    chosen, n = [], 0
+    for o in cases:
+        chosen.append(o)
+        n += o.nx * o.ny * o.nz
+        if n >= min_cells:
+            break
+    rate, cells, secs = _oracle_rate(chosen, steps)
+    return {"value": rate, "unit": "cell-stage updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (single-thread C++ restatement of the reference's serial path), "
+                      f"the first {len(chosen)} cases of the batch ({cells} cells), {steps} RK4 steps "
+                      f"each, {secs:.1f} s"}
 
 
 def load_traffic(workload: str, kernel: str):
@@ -82,6 +116,154 @@ def load_traffic(workload: str, kernel: str):
             return float(json.load(f)["hbm_bytes_per_launch"])
     except Exception:
         return None
+
+
+def precision_errors(U, R) -> dict:
+    """Relative L2 of U against the FP64 state R: whole state and per
+    variable, momentum components scaled by the momentum-vector norm."""
+    import numpy as np
+    mom = np.linalg.norm(R[:, 1:4])
+    scale = [np.linalg.norm(R[:, 0]), mom, mom, mom, np.linalg.norm(R[:, 4])]
+    names = ("rho", "rho_u", "rho_v", "rho_w", "rho_E")
+    return {"whole_state": float(np.linalg.norm(U - R) / np.linalg.norm(R)),
+            **{n: float(np.linalg.norm(U[:, k] - R[:, k]) / scale[k]) for k, n in enumerate(names)}}
+
+
+def timed_run(solver, steps: int, warmup: int):
+    solver.initialize()
+    solver.step(warmup)
+    solver.synchronize()
+    t0 = time.perf_counter()
+    solver.step(steps)
+    solver.synchronize()
+    return time.perf_counter() - t0
+
+
+def roofline(solver, profile_steps, nc, nif, nbf, so, fp32, traffic, copy_gbs):
+    prof = solver.profile(profile_steps)
+    t = g = 4 if fp32 else 8
+    dom = max(prof, key=lambda k: prof[k][0] * prof[k][1])
+    launches, mean_ms = prof[dom]
+    alg = algorithmic_bytes(dom, nc, nif, nbf, so, t, g)
+    achieved = alg / (mean_ms * 1e-3) / 1e9
+    kernels = {k: {"launches": v[0], "mean_ms": round(v[1], 5),
+                   "GBps": round(algorithmic_bytes(k, nc, nif, nbf, so, t, g) / (v[1] * 1e-3) / 1e9, 1)}
+               for k, v in prof.items()}
+    print(f"[bench] kernels: {json.dumps(kernels)}", file=sys.stderr, flush=True)
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic(dom),
+            "kernel": dom, "algorithmic_bytes_per_launch": alg,
+            "mean_launch_ms": round(mean_ms, 5), "copy_ceiling_GBps": round(copy_gbs, 1),
+            "kernels": kernels}
+
+
+def bench_single(args, M, opts, copy_gbs) -> dict:
+    t0 = time.perf_counter()
+    md = M.Parallel3DMesh(opts.nx, opts.ny, opts.nz, opts.lx, opts.ly, opts.lz, opts.problem_type,
+                          opts.angle, options=opts).fillMeshData()
+    solver = M.TimeSolverExplicitRK4(md, opts)
+    setup_s = time.perf_counter() - t0
+    sizes = solver.sizes()
+    nc, nif, nbf = sizes["cells"], sizes["internal_faces"], sizes["boundary_faces"]
+    print(f"[bench] {args.workload}: {nc} cells, {nif} internal + {nbf} boundary faces, "
+          f"setup {setup_s:.1f} s, device {sizes['device_bytes'] / 2**30:.2f} GiB", file=sys.stderr,
+          flush=True)
+    elapsed = timed_run(solver, args.steps, args.warmup)
+    if solver.nonfinite():
+        raise RuntimeError("non-finite state after the timed steps")
+    print(f"[bench] {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    extra = {}
+    if args.compare_fp64 and opts.precision != 0:
+        # same steps from the same start in FP64; error of the timed run's state
+        U = solver.state()
+        ref = M.TimeSolverExplicitRK4(md, opts.replace(precision=0))
+        ref.initialize()
+        ref.step(args.warmup + args.steps)
+        err = precision_errors(U, ref.state())
+        ref.close()
+        extra["precision_error"] = {
+            "metric": "relative L2 vs the FP64 run after warmup+steps; momentum components "
+                      "scaled by the momentum-vector norm", "steps": args.warmup + args.steps, **err}
+        print(f"[bench] precision error: {json.dumps(extra['precision_error'])}", file=sys.stderr, flush=True)
+
+    so = bool(opts.second_order_space)
+    roof = roofline(solver, args.profile_steps, nc, nif, nbf, so, opts.precision != 0,
+                    lambda k: load_traffic(args.workload, k), copy_gbs)
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(opts)
+        except Exception as e:  # the oracle is optional on a machine without a compiler
+            print(f"[bench] cpu baseline skipped: {e}", file=sys.stderr)
+    _, dt_last = solver.time()
+    solver.close()
+    physics = ("viscous " if opts.viscous else "inviscid ") + "Roe, " + (
+        ("Green-Gauss, " + ("Van Albada" if opts.limiter else "Venkat") + ", 2nd order") if so else "1st order")
+    config = {"workload": args.workload, "mesh": f"{opts.nx}x{opts.ny}x{opts.nz}", "cells": nc,
+              "internal_faces": nif, "boundary_faces": nbf, "physics": physics,
+              "cfl": opts.cfl, "last_dt": dt_last,
+              "precision": "fp64" if opts.precision == 0 else "fp32 flux/gradient/limiter, fp64 state"}
+    return dict(value=nc * 4 * args.steps / elapsed, elapsed=elapsed, config=config, roofline=roof,
+                cpu_baseline=cpu, fp32=opts.precision != 0, extra=extra)
+
+
+def bench_batch(args, M, cases, copy_gbs) -> dict:
+    import numpy as np
+    t0 = time.perf_counter()
+    from miniaero_amd.mesh import generate_meshes
+    meshes = generate_meshes(cases)
+    gen_s = time.perf_counter() - t0
+    solver = M.BatchSolver(list(zip(meshes, cases)))
+    setup_s = time.perf_counter() - t0
+    sizes = solver.sizes()
+    nc, nif, nbf = sizes["cells"], sizes["internal_faces"], sizes["boundary_faces"]
+    print(f"[bench] {args.workload}: {len(cases)} cases, {nc} cells, {nif} internal + {nbf} boundary "
+          f"faces, mesh generation {gen_s:.1f} s, setup {setup_s:.1f} s, device "
+          f"{sizes['device_bytes'] / 2**30:.2f} GiB", file=sys.stderr, flush=True)
+    elapsed = timed_run(solver, args.steps, args.warmup)
+    if solver.nonfinite():
+        raise RuntimeError("non-finite state after the timed steps")
+    print(f"[bench] {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    extra = {}
+    fp32 = cases[0].precision != 0
+    if args.compare_fp64 and fp32:
+        U = [solver.case_state(k) for k in range(len(cases))]
+        ref = M.BatchSolver(list(zip(meshes, [o.replace(precision=0) for o in cases])))
+        ref.initialize()
+        ref.step(args.warmup + args.steps)
+        errs = [precision_errors(U[k], ref.case_state(k)) for k in range(len(cases))]
+        ref.close()
+        worst = {k: max(e[k] for e in errs) for k in errs[0]}
+        extra["precision_error"] = {
+            "metric": "relative L2 vs the FP64 run after warmup+steps, worst case of the batch; "
+                      "momentum components scaled by the momentum-vector norm",
+            "steps": args.warmup + args.steps, **worst}
+        print(f"[bench] precision error: {json.dumps(extra['precision_error'])}", file=sys.stderr, flush=True)
+    del meshes
+    roof = roofline(solver, args.profile_steps, nc, nif, nbf, True, fp32,
+                    lambda k: load_traffic(args.workload, k), copy_gbs)
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline_batch(cases)
+        except Exception as e:
+            print(f"[bench] cpu baseline skipped: {e}", file=sys.stderr)
+    solver.close()
+    cells = np.array([o.nx * o.ny * o.nz for o in cases])
+    # every case of the batch completes with the batch: its latency is the
+    # time of the workload's step count at the measured step time
+    per_case_ms = elapsed / args.steps * cases[0].ntimesteps * 1e3
+    config = {"workload": args.workload, "cases": len(cases), "cells": nc,
+              "cells_per_case": {"min": int(cells.min()), "mean": float(cells.mean()), "max": int(cells.max())},
+              "internal_faces": nif, "boundary_faces": nbf,
+              "physics": "inviscid Roe, Green-Gauss, Van Albada, 2nd order, Sod-type states",
+              "dt": cases[0].dt, "steps_per_case": cases[0].ntimesteps,
+              "precision": "fp32 flux/gradient/limiter, fp64 state" if fp32 else "fp64",
+              "case_latency_ms": {"p50": round(per_case_ms, 2), "p99": round(per_case_ms, 2),
+                                  "note": "all cases run in the same launches and finish together"}}
+    return dict(value=nc * 4 * args.steps / elapsed, elapsed=elapsed, config=config, roofline=roof,
+                cpu_baseline=cpu, fp32=fp32, extra=extra)
 
 
 def main() -> int:
@@ -106,7 +288,7 @@ def main() -> int:
 
     import miniaero_amd as M
     from miniaero_amd import _lib
-    from miniaero_amd.configs import WORKLOADS
+    from miniaero_amd.configs import BATCH_WORKLOADS, WORKLOADS
 
     copy_bytes = 1 << 30
     copy_ms = _lib.measure_copy(copy_bytes, 10)
@@ -115,104 +297,38 @@ def main() -> int:
         print(json.dumps({"copy_bytes": copy_bytes, "ms_per_copy": copy_ms, "GBps_read_plus_write": copy_gbs}))
         return 0
 
-    opts = WORKLOADS[args.workload]()
-    if args.precision is not None:
-        opts = opts.replace(precision=args.precision)
-    t0 = time.perf_counter()
-    md = M.Parallel3DMesh(opts.nx, opts.ny, opts.nz, opts.lx, opts.ly, opts.lz, opts.problem_type,
-                          opts.angle, options=opts).fillMeshData()
-    solver = M.TimeSolverExplicitRK4(md, opts)
-    setup_s = time.perf_counter() - t0
-    sizes = solver.sizes()
-    nc, nif, nbf = sizes["cells"], sizes["internal_faces"], sizes["boundary_faces"]
-    print(f"[bench] {args.workload}: {nc} cells, {nif} internal + {nbf} boundary faces, "
-          f"setup {setup_s:.1f} s, device {sizes['device_bytes'] / 2**30:.2f} GiB", file=sys.stderr,
-          flush=True)
-
-    solver.initialize()
-    solver.step(args.warmup)
-    solver.synchronize()
-    t0 = time.perf_counter()
-    solver.step(args.steps)
-    solver.synchronize()
-    elapsed = time.perf_counter() - t0
-    if solver.nonfinite():
-        print("[bench] non-finite state after the timed steps", file=sys.stderr)
+    try:
+        if args.workload in BATCH_WORKLOADS:
+            cases = BATCH_WORKLOADS[args.workload]()
+            if args.precision is not None:
+                cases = [o.replace(precision=args.precision) for o in cases]
+            r = bench_batch(args, M, cases, copy_gbs)
+        else:
+            opts = WORKLOADS[args.workload]()
+            if args.precision is not None:
+                opts = opts.replace(precision=args.precision)
+            r = bench_single(args, M, opts, copy_gbs)
+    except RuntimeError as e:
+        print(f"[bench] {e}", file=sys.stderr)
         return 1
-    value = nc * 4 * args.steps / elapsed
-    print(f"[bench] {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
-
-    precision_error = None
-    if args.compare_fp64 and opts.precision != 0:
-        # same steps from the same start in FP64; error of the timed run's state
-        import numpy as np
-        U = solver.state()
-        ref = M.TimeSolverExplicitRK4(md, opts.replace(precision=0))
-        ref.initialize()
-        ref.step(args.warmup + args.steps)
-        R = ref.state()
-        ref.close()
-        mom = np.linalg.norm(R[:, 1:4])
-        scale = [np.linalg.norm(R[:, 0]), mom, mom, mom, np.linalg.norm(R[:, 4])]
-        names = ("rho", "rho_u", "rho_v", "rho_w", "rho_E")
-        precision_error = {
-            "metric": "relative L2 vs the FP64 run after warmup+steps; momentum components "
-                      "scaled by the momentum-vector norm",
-            "steps": args.warmup + args.steps,
-            "whole_state": float(np.linalg.norm(U - R) / np.linalg.norm(R)),
-            **{n: float(np.linalg.norm(U[:, k] - R[:, k]) / scale[k]) for k, n in enumerate(names)},
-            "raw_per_component": {n: float(np.linalg.norm(U[:, k] - R[:, k]) / max(np.linalg.norm(R[:, k]), 1e-300))
-                                  for k, n in enumerate(names)}}
-        print(f"[bench] precision error: {json.dumps(precision_error)}", file=sys.stderr, flush=True)
-
-    prof = solver.profile(args.profile_steps)
-    so = bool(opts.second_order_space)
-    dom = max(prof, key=lambda k: prof[k][0] * prof[k][1])
-    launches, mean_ms = prof[dom]
-    alg = algorithmic_bytes(dom, nc, nif, nbf, so)
-    achieved = alg / (mean_ms * 1e-3) / 1e9
-    kernels = {k: {"launches": v[0], "mean_ms": round(v[1], 5),
-                   "GBps": round(algorithmic_bytes(k, nc, nif, nbf, so) / (v[1] * 1e-3) / 1e9, 1)}
-               for k, v in prof.items()}
-    print(f"[bench] kernels: {json.dumps(kernels)}", file=sys.stderr, flush=True)
-    traffic = load_traffic(args.workload, dom)
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        try:
-            cpu = cpu_baseline(opts)
-        except Exception as e:  # the oracle is optional on a machine without a compiler
-            print(f"[bench] cpu baseline skipped: {e}", file=sys.stderr)
-
-    t_final, dt_last = solver.time()
-    solver.close()
     line = {
         "metric": "RK4 cell-stage updates/s",
-        "value": value,
+        "value": r["value"],
         "unit": "cell-stage updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": r["elapsed"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if opts.precision == 0 else "f32 arithmetic, f64 state",
+        "dtype": "f32 arithmetic, f64 state" if r["fp32"] else "f64",
         "data": "synthetic",
-        "config": {"workload": args.workload, "mesh": f"{opts.nx}x{opts.ny}x{opts.nz}", "cells": nc,
-                   "internal_faces": nif, "boundary_faces": nbf,
-                   "physics": "viscous Roe, Green-Gauss, Venkat, 2nd order" if so else "first order",
-                   "cfl": opts.cfl, "last_dt": dt_last,
-                   "precision": "fp64" if opts.precision == 0 else "fp32 flux/gradient/limiter, fp64 state"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": dom, "algorithmic_bytes_per_launch": alg,
-                     "mean_launch_ms": round(mean_ms, 5), "copy_ceiling_GBps": round(copy_gbs, 1),
-                     "kernels": kernels},
-        "cpu_baseline": cpu,
+        "config": r["config"],
+        "roofline": r["roofline"],
+        "cpu_baseline": r["cpu_baseline"],
     }
-    if precision_error is not None:
-        line["precision_error"] = precision_error
+    line.update(r["extra"])
     print(json.dumps(line), flush=True)
     return 0
 

@@ -167,6 +167,29 @@ ma_status ma_solver_check_finite(ma_solver* s, long long* nonfinite);
 ma_status ma_solver_stream(ma_solver* s, void** stream);
 void ma_solver_destroy(ma_solver* s);
 
+/* ---- batched form: independent cases stepped together ---- */
+
+/* One run over ncases independent meshes (the reference runs one mesh per
+ * process, Main.C:96-141; no counterpart).  Case k has its own mesh, initial
+ * state, inflow state and dt or CFL number; all cases must share
+ * second_order_space, viscous, limiter, precision and prandtl.  Every step
+ * advances every case in the same launches; the result of each case is bit
+ * for bit that of ma_solver_create on its mesh and options alone.
+ * ma_solver_* calls apply to the whole batch: states, residuals and
+ * gradients are the cases' cells in order, ma_solver_get_time reports case 0. */
+ma_status ma_batch_create(int ncases, const ma_mesh_data* const* meshes, const ma_options* opts,
+                          ma_solver** out);
+/* Number of cases and, if case_cell_offsets is not NULL, the ncases + 1
+ * offsets of each case's first cell in the batch's cell numbering. */
+ma_status ma_batch_sizes(ma_solver* s, int* ncases, int* case_cell_offsets);
+/* ma_mesh_plan over the concatenated layout ma_batch_create would build
+ * (lattice dimensions are 0 for more than one case).  No reference counterpart. */
+ma_status ma_batch_plan(int ncases, const ma_mesh_data* const* meshes, long long* stats);
+/* State of case k alone (its num_owned_cells x 5 conserved values). */
+ma_status ma_batch_get_case_state(ma_solver* s, int k, double* U);
+/* Time and last dt of case k (dt differs between CFL-driven cases). */
+ma_status ma_batch_get_case_time(ma_solver* s, int k, double* time, double* last_dt);
+
 /* ---- measurement ---- */
 
 /* Per-kernel timing: runs nsteps with a HIP event pair around every launch on

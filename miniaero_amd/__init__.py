@@ -10,13 +10,13 @@ mirror of the reference's interfaces:
     results / numeric_text_diff  results.<rank> files and their comparator
 """
 from ._lib import MiniAeroError, load
-from .mesh import Faces, MeshData, Parallel3DMesh
+from .mesh import Faces, MeshData, Parallel3DMesh, plan_batch
 from .options import Options
 from .results import numeric_file_diff, numeric_text_diff, read_results, write_results
-from .solver import TimeSolverExplicitRK4
+from .solver import BatchSolver, TimeSolverExplicitRK4
 
 __all__ = [
     "MiniAeroError", "load", "Faces", "MeshData", "Parallel3DMesh", "Options",
-    "TimeSolverExplicitRK4", "numeric_text_diff", "numeric_file_diff", "read_results",
+    "TimeSolverExplicitRK4", "BatchSolver", "numeric_text_diff", "numeric_file_diff", "read_results",
     "write_results",
 ]

@@ -31,7 +31,8 @@ EXPORTS = (
     "ma_solver_synchronize", "ma_solver_get_state", "ma_solver_get_residual",
     "ma_solver_get_gradients", "ma_solver_get_time", "ma_solver_check_finite",
     "ma_solver_stream", "ma_solver_destroy", "ma_solver_profile", "ma_solver_sizes",
-    "ma_measure_copy", "ma_mesh_plan",
+    "ma_measure_copy", "ma_mesh_plan", "ma_batch_create", "ma_batch_sizes", "ma_batch_get_case_state",
+    "ma_batch_get_case_time", "ma_batch_plan",
 )
 
 
@@ -136,6 +137,13 @@ def load() -> ctypes.CDLL:
     L.ma_solver_sizes.argtypes = [P, ctypes.POINTER(ctypes.c_longlong)]
     L.ma_measure_copy.argtypes = [ctypes.c_longlong, ctypes.c_int, _D]
     L.ma_mesh_plan.argtypes = [ctypes.POINTER(ma_mesh_data), ctypes.POINTER(ctypes.c_longlong)]
+    L.ma_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.POINTER(ma_mesh_data)),
+                                  ctypes.POINTER(ma_options), PP]
+    L.ma_batch_plan.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.POINTER(ma_mesh_data)),
+                                ctypes.POINTER(ctypes.c_longlong)]
+    L.ma_batch_sizes.argtypes = [P, _I, _I]
+    L.ma_batch_get_case_state.argtypes = [P, ctypes.c_int, _D]
+    L.ma_batch_get_case_time.argtypes = [P, ctypes.c_int, _D, _D]
     _lib = L
     return L
 

@@ -10,6 +10,8 @@ from __future__ import annotations
 
 import math
 
+import numpy as np
+
 from .options import Options
 
 RGAS = 287.05    # GasModel.h:39
@@ -64,4 +66,29 @@ def flat_plate_c3(**kw) -> Options:
                          precision=1).replace(**kw)
 
 
+def batched_c4(ncases: int = 512, seed: int = 4, **kw) -> list:
+    """Independent Sod-type cases stepped together: cell counts N drawn
+    log-uniform over [8192, 262144], mesh 4m x m x m with m = round((N/4)^(1/3))
+    on [0,1]x[0,0.25]x[0,0.25]; left/right density and pressure U[0.1, 1],
+    interface position U[0.3, 0.7]; Extrapolate on every side, second order,
+    Van Albada, dt 1e-4 (largest sound speed sqrt(1.4 * 1 / 0.1) = 3.74 on
+    dx >= 1/160 gives a Courant number below 0.06), 100 steps.  Draws from
+    numpy's default_rng(seed) in the order N, rho_L, rho_R, p_L, p_R, x0."""
+    rng = np.random.default_rng(seed)
+    n = rng.uniform(math.log(8192.0), math.log(262144.0), ncases)
+    rho_l, rho_r = rng.uniform(0.1, 1.0, ncases), rng.uniform(0.1, 1.0, ncases)
+    p_l, p_r = rng.uniform(0.1, 1.0, ncases), rng.uniform(0.1, 1.0, ncases)
+    x0 = rng.uniform(0.3, 0.7, ncases)
+    out = []
+    for i in range(ncases):
+        m = max(1, int(round((math.exp(n[i]) / 4.0) ** (1.0 / 3.0))))
+        o = Options(problem_type=0, lx=1.0, ly=0.25, lz=0.25, nx=4 * m, ny=m, nz=m, ntimesteps=100,
+                    dt=1e-4, second_order_space=1, viscous=0, limiter=1, ic_kind=1,
+                    ic=(float(rho_l[i]), float(p_l[i]), float(rho_r[i]), float(p_r[i]), float(x0[i])),
+                    bc_kind=(1, 1, 1, 1, 1, 1))
+        out.append(o.replace(**kw))
+    return out
+
+
 WORKLOADS = {"sod_c1": sod_c1, "flat_plate_c2": flat_plate_c2, "flat_plate_c3": flat_plate_c3}
+BATCH_WORKLOADS = {"batched_c4": batched_c4}

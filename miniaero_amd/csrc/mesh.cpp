@@ -18,7 +18,8 @@
  *     binormal are evaluated with the operation order of Face.C:57-96;
  *   - internal faces keep creation order and are then permuted by
  *     std::random_shuffle from the C library's default rand() state
- *     (Parallel3DMesh.h:362); srand(1) restores that state;
+ *     (Parallel3DMesh.h:362), reproduced by a private srand(1)-seeded state
+ *     so that generation is thread-safe;
  *   - the processing order (Faces.h:128-144) is a stable sort of the shuffled
  *     faces by left cell, done here as a counting sort;
  *   - boundary faces are grouped per side in cell order, in the group order
@@ -301,8 +302,25 @@ extern "C" ma_status ma_mesh_generate(const ma_options* o, ma_mesh** out) {
           if (j + 1 < ny) keys.push_back(c * 8 + 2);
           if (k + 1 < nz) keys.push_back(c * 8 + 5);
         }
-    std::srand(1);
-    std::random_shuffle(keys.begin(), keys.end());
+    // The C library's rand() after srand(1), drawn from a private state of
+    // the same generator (random_r over a 128-byte state, as rand() uses) so
+    // that meshes can be generated on several threads at once; rand() is
+    // random(), and random_shuffle(first, last) draws rand() % (i + 1)
+    // exactly as the functor form does with this functor.
+    struct random_data rd;
+    std::memset(&rd, 0, sizeof rd);
+    char rstate[128];
+    std::memset(rstate, 0, sizeof rstate);
+    if (initstate_r(1u, rstate, sizeof rstate, &rd) != 0 || srandom_r(1u, &rd) != 0) {
+      delete m;
+      return ma::fail(MA_ERR_INVALID, "mesh generator: random state initialisation failed");
+    }
+    auto draw = [&rd](long long n) {
+      int32_t r = 0;
+      random_r(&rd, &r);
+      return (long long)(r % n);
+    };
+    std::random_shuffle(keys.begin(), keys.end(), draw);
     const int nf = (int)keys.size();
     FaceSet& F = m->internal;
     F.resize(nf);

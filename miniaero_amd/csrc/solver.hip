@@ -43,12 +43,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/miniaero.h"
@@ -95,6 +97,7 @@ struct TileMap {
   const int* __restrict__ faces;  // device face indices
   const int* __restrict__ slot;   // kMaxSlots planes of nc, see k_flux
   const int* __restrict__ cptr;   // ntiles + 1: cells of tile b are [cptr[b], cptr[b+1])
+  const int* __restrict__ tcase;  // ntiles: the case (independent mesh) tile b belongs to
 };
 
 template <class S, class T>
@@ -152,6 +155,7 @@ struct IcParams {
   double inflow[5];
   double perturb;
   unsigned long long seed;
+  int cell0;         // first cell of the case in the caller's union numbering
 };
 
 __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
@@ -166,9 +170,12 @@ __device__ __forceinline__ double unif(unsigned long long seed, long long cell, 
   return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_init(DevMesh m, IcParams p, double* __restrict__ U) {
+__global__ __launch_bounds__(kBlock) void k_init(DevMesh m, const IcParams* __restrict__ cases,
+                                                 const int* __restrict__ cell_case,
+                                                 double* __restrict__ U) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= m.nc) return;
+  const IcParams& p = cases[cell_case[c]];
   const double Rgas = 287.05, gamma = 1.4;
   const double x = m.xc[c];
   double u[5];
@@ -186,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void k_init(DevMesh m, IcParams p, double* 
     ma::primitives(p.inflow, V);
     const double speed = sqrt(V[1] * V[1] + V[2] * V[2] + V[3] * V[3]);
     const double pr = V[0] * Rgas * V[4];
-    const int oc = m.org[c];  // the draw is keyed by the caller's cell index
+    const int oc = m.org[c] - p.cell0;  // the draw is keyed by the case's own cell index
     const double rho = V[0] * (1.0 + p.perturb * unif(p.seed, oc, 0));
     const double uu = V[1] + p.perturb * speed * unif(p.seed, oc, 1);
     const double vv = V[2] + p.perturb * speed * unif(p.seed, oc, 2);
@@ -415,7 +422,8 @@ __device__ __forceinline__ void interior_flux(const DevMesh& m, const T* __restr
 // TimeSolverExplicitRK4.h:457-460).  Written to bc[5][nbf].
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const T* __restrict__ V, double prandtl,
-                                               Inflow inflow, T* __restrict__ bc) {
+                                               const Inflow* __restrict__ inflows,
+                                               const int* __restrict__ bc_case, T* __restrict__ bc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int nbf = m.nf - m.nif;
   if (i >= nbf) return;
@@ -433,7 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const T* __restrict__ 
     for (int k = 0; k < 5; ++k) out[k] = -F[k];
   } else if (kind == MA_BC_INFLOW) {
     double vi[5];
-    ma::primitives(inflow.U, vi);
+    ma::primitives(inflows[bc_case[i]].U, vi);
 #pragma unroll
     for (int k = 0; k < 5; ++k) vr[k] = T(vi[k]);
     roe.flux_frame(vc, vr, F, fr);
@@ -550,7 +558,7 @@ __global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const T* _
         for (int k = 0; k < 5; ++k) acc[k] += double(bc[(size_t)k * nbf + j]);
       }
   }
-  const double dt = *dt_dev;
+  const double dt = dt_dev[t.tcase[tile]];
   const double vol = m.vol[c];
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
@@ -566,13 +574,14 @@ __global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const T* _
 //   dt_c = vol / (lc + 4 kappa lv / vol),  kappa = max(4/3, gamma/Pr) mu / rho
 // (kappa = 0 for inviscid runs: convective plus viscous spectral radius),
 // reduced to the minimum per block.
-__global__ __launch_bounds__(kBlock) void k_dt_partial(DevMesh m, const double* __restrict__ Un,
-                                                       int viscous, double prandtl,
-                                                       double* __restrict__ partial) {
-  __shared__ double red[kBlock / 64];
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kTile) void k_dt_partial(DevMesh m, TileMap t,
+                                                      const double* __restrict__ Un, int viscous,
+                                                      double prandtl, double* __restrict__ partial) {
+  __shared__ double red[kTile / 64];
+  const int tile = blockIdx.x;  // one workgroup per tile: a tile never spans two cases
+  const int c = t.cptr[tile] + threadIdx.x;
   double dtc = 1e300;
-  if (c < m.nc) {
+  if (c < t.cptr[tile + 1]) {
     const int nc = m.nc, nf = m.nf;
     double u[5], v[5];
     load5(Un, nc, c, u);
@@ -606,17 +615,26 @@ __global__ __launch_bounds__(kBlock) void k_dt_partial(DevMesh m, const double* 
   __syncthreads();
   if (threadIdx.x == 0) {
     double x = red[0];
-    for (int w = 1; w < kBlock / 64; ++w) x = ma::smin(x, red[w]);
-    partial[blockIdx.x] = x;
+    for (int w = 1; w < kTile / 64; ++w) x = ma::smin(x, red[w]);
+    partial[tile] = x;
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_dt_final(const double* __restrict__ partial, int np,
-                                                     double cfl, double* __restrict__ dt_dev,
+// One workgroup per case: dt = cfl * (minimum over the case's tiles) when the
+// case runs at a CFL number (cfl > 0), else its fixed dt; then time += dt
+// (TimeSolverExplicitRK4.h:343).
+__global__ __launch_bounds__(kBlock) void k_dt_final(const double* __restrict__ partial,
+                                                     const int* __restrict__ case_tiles,
+                                                     const double* __restrict__ cfl,
+                                                     double* __restrict__ dt_dev,
                                                      double* __restrict__ time_dev) {
   __shared__ double red[kBlock / 64];
+  const int k = blockIdx.x;
+  const double cf = cfl[k];
   double x = 1e300;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) x = ma::smin(x, partial[i]);
+  if (cf > 0.0)
+    for (int i = case_tiles[k] + threadIdx.x; i < case_tiles[k + 1]; i += blockDim.x)
+      x = ma::smin(x, partial[i]);
   for (int off = 32; off > 0; off >>= 1) x = ma::smin(x, __shfl_down(x, off, 64));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) red[wave] = x;
@@ -624,9 +642,9 @@ __global__ __launch_bounds__(kBlock) void k_dt_final(const double* __restrict__ 
   if (threadIdx.x == 0) {
     double y = red[0];
     for (int w = 1; w < kBlock / 64; ++w) y = ma::smin(y, red[w]);
-    const double dt = cfl * y;
-    *dt_dev = dt;
-    *time_dev = *time_dev + dt;
+    const double dt = cf > 0.0 ? cf * y : dt_dev[k];
+    dt_dev[k] = dt;
+    time_dev[k] = time_dev[k] + dt;
   }
 }
 
@@ -680,11 +698,19 @@ struct ma_solver {
   // gradients, limiters, boundary contributions.
   void *V = nullptr, *grad = nullptr, *phi = nullptr, *bc = nullptr;
   bool fp32 = false;
-  double *dt_dev = nullptr, *time_dev = nullptr, *partial = nullptr;
+  // Cases (independent meshes stepped together; a plain run is one case).
+  int ncases = 1;
+  std::vector<ma_options> case_opt;
+  std::vector<int> case_cell_off;  // ncases + 1: cells of case k in the caller's union order
+  bool any_cfl = false;
+  double *dt_dev = nullptr, *time_dev = nullptr, *cfl_dev = nullptr;  // per case
+  double* partial = nullptr;                                          // per tile
+  int *cell_case = nullptr, *bc_case = nullptr, *case_tiles = nullptr;
+  IcParams* ic_dev = nullptr;
+  Inflow* inflow_dev = nullptr;
   unsigned long long* counter = nullptr;
   int cur = 0;  // U[cur] holds sol_n
-  double time = 0.0, last_dt = 0.0;
-  Inflow inflow{};
+  long long steps_done = 0;
   Timing timing;
 
   template <class T>
@@ -853,6 +879,8 @@ void cell_order(const ma_mesh_data* md, std::vector<int>& dev_of, std::vector<in
 
 // Everything the device needs, computed on the host (no GPU involved).
 struct Layout {
+  // frees every array of `other` (used to release merged parts)
+  void swap_out(Layout& other) { std::swap(*this, other); }
   int nc = 0, nif = 0, nf = 0, nbf = 0, ntiles = 0, max_faces = 1;
   int dims[3] = {0, 0, 0};  // mesh lattice, zeros when there is none
   bool derive_frame = true;  // every face's t and b follow from its area vector
@@ -860,6 +888,147 @@ struct Layout {
   std::vector<double> fn, ffr, fx, fwall, vol, xc;
   std::vector<signed char> fkind;
 };
+
+// Every index a kernel will use, checked on the host before any upload.
+ma_status validate_layout(const Layout& L) {
+  const int nc = L.nc, nif = L.nif, nf = L.nf, nbf = L.nbf, ntiles = L.ntiles;
+  const std::vector<int>& dev_of = L.dev_of;
+  const std::vector<int>& tile_cptr = L.tile_cptr;
+  const std::vector<int>& tptr = L.tptr;
+  const std::vector<int>& tfaces = L.tfaces;
+  const std::vector<int>& fl = L.fl;
+  const std::vector<int>& fr = L.fr;
+  const std::vector<int>& slot = L.slot;
+  const std::vector<int>& tslot = L.tslot;
+  if ((int)tile_cptr.size() != ntiles + 1 || (int)tptr.size() != ntiles + 1 || (int)dev_of.size() != nc)
+    return ma::fail(MA_ERR_INVALID, "layout: sizes");
+  if ((size_t)L.max_faces * 5 * sizeof(double) > 65536)
+    return ma::fail(MA_ERR_UNSUPPORTED, "too many faces per cell tile for the flux kernel's LDS");
+  std::vector<int> tile_of(nc, -1);
+  for (int b = 0; b < ntiles; ++b)
+    for (int d = tile_cptr[b]; d < tile_cptr[b + 1] && d < nc; ++d) tile_of[d] = b;
+  for (int b = 0; b < ntiles; ++b)
+    if (tptr[b + 1] - tptr[b] > L.max_faces) return ma::fail(MA_ERR_INVALID, "layout: tile faces");
+  {
+  std::vector<char> hit(nc, 0);
+  for (int c = 0; c < nc; ++c) {
+    if (dev_of[c] < 0 || dev_of[c] >= nc || hit[dev_of[c]]) return ma::fail(MA_ERR_INVALID, "layout: cell order");
+    hit[dev_of[c]] = 1;
+  }
+  if (tile_cptr.front() != 0 || tile_cptr.back() != nc) return ma::fail(MA_ERR_INVALID, "layout: tile cells");
+  for (int b = 0; b < ntiles; ++b) {
+    const int n = tile_cptr[b + 1] - tile_cptr[b];
+    if (n < 1 || n > kTile) return ma::fail(MA_ERR_INVALID, "layout: tile size");
+    for (int q = tptr[b]; q < tptr[b + 1]; ++q)
+      if (tfaces[q] < 0 || tfaces[q] >= nif) return ma::fail(MA_ERR_INVALID, "layout: tile face");
+  }
+  for (int f = 0; f < nf; ++f)
+    if (fl[f] < 0 || fl[f] >= nc || (f < nif && (fr[f] < 0 || fr[f] >= nc)))
+      return ma::fail(MA_ERR_INVALID, "layout: face cells");
+  for (int c = 0; c < nc; ++c) {
+    const int b = tile_of[c];
+    if (b < 0) return ma::fail(MA_ERR_INVALID, "layout: cell outside every tile");
+    const int nloc = tptr[b + 1] - tptr[b];
+    for (int q = 0; q < kMaxSlots; ++q) {
+      const int e = tslot[(size_t)q * nc + c], g = slot[(size_t)q * nc + c];
+      if ((e < 0) != (g < 0)) return ma::fail(MA_ERR_INVALID, "layout: slot tables");
+      if (e < 0) continue;
+      if ((g >> 1) >= nf) return ma::fail(MA_ERR_INVALID, "layout: slot face");
+      if ((e & 7) < 2 ? (e >> 3) >= nloc || tfaces[tptr[b] + (e >> 3)] != (g >> 1)
+                      : (e >> 3) >= nbf)
+        return ma::fail(MA_ERR_INVALID, "layout: tile slot");
+    }
+  }
+  }
+  return MA_OK;
+}
+
+// Concatenation of independent cases' layouts into one: cells, internal faces,
+// boundary faces and tiles of case k follow those of case k-1; tiles never
+// span two cases.  Internal faces of all cases come before all boundary faces.
+void merge_layouts(std::vector<Layout>& parts, Layout& M, std::vector<int>& tile_case,
+                   std::vector<int>& case_tiles, std::vector<int>& bc_case, std::vector<int>& cell_case) {
+  const int n = (int)parts.size();
+  std::vector<int> co(n + 1, 0), io(n + 1, 0), bo(n + 1, 0), to(n + 1, 0), qo(n + 1, 0);
+  M.derive_frame = true;
+  M.max_faces = 1;
+  for (int k = 0; k < n; ++k) {
+    const Layout& L = parts[k];
+    co[k + 1] = co[k] + L.nc;
+    io[k + 1] = io[k] + L.nif;
+    bo[k + 1] = bo[k] + L.nbf;
+    to[k + 1] = to[k] + L.ntiles;
+    qo[k + 1] = qo[k] + L.tptr[L.ntiles];
+    M.derive_frame = M.derive_frame && L.derive_frame;
+    M.max_faces = std::max(M.max_faces, L.max_faces);
+  }
+  M.nc = co[n];
+  M.nif = io[n];
+  M.nbf = bo[n];
+  M.nf = M.nif + M.nbf;
+  M.ntiles = to[n];
+  const int nc = M.nc, nf = M.nf;
+  M.dev_of.assign(nc, 0);
+  M.org_of.assign(nc, 0);
+  M.tile_cptr.assign(M.ntiles + 1, 0);
+  M.tptr.assign(M.ntiles + 1, 0);
+  M.tfaces.assign(std::max(1, qo[n]), 0);
+  M.slot.assign((size_t)kMaxSlots * nc, -1);
+  M.tslot.assign((size_t)kMaxSlots * nc, -1);
+  M.fl.assign(nf, 0);
+  M.fr.assign(nf, -1);
+  M.fkind.assign(nf, 0);
+  M.fwall.assign(nf, 0.0);
+  M.fn.assign((size_t)3 * nf, 0.0);
+  M.ffr.assign((size_t)10 * nf, 0.0);
+  M.fx.assign((size_t)3 * nf, 0.0);
+  M.vol.assign(nc, 0.0);
+  M.xc.assign((size_t)3 * nc, 0.0);
+  tile_case.assign(M.ntiles, 0);
+  case_tiles.assign(n + 1, 0);
+  bc_case.assign(std::max(1, M.nbf), 0);
+  cell_case.assign(nc, 0);
+  for (int k = 0; k < n; ++k) {
+    Layout& L = parts[k];
+    const int lnc = L.nc, lnif = L.nif, lnf = L.nf;
+    auto gf = [&](int f) { return f < lnif ? io[k] + f : M.nif + bo[k] + (f - lnif); };
+    for (int c = 0; c < lnc; ++c) {
+      M.dev_of[co[k] + c] = co[k] + L.dev_of[c];
+      M.org_of[co[k] + c] = co[k] + L.org_of[c];
+      M.vol[co[k] + c] = L.vol[c];
+      cell_case[co[k] + c] = k;
+      for (int d = 0; d < 3; ++d) M.xc[(size_t)d * nc + co[k] + c] = L.xc[(size_t)d * lnc + c];
+      for (int q = 0; q < kMaxSlots; ++q) {
+        const int e = L.slot[(size_t)q * lnc + c];
+        if (e >= 0) M.slot[(size_t)q * nc + co[k] + c] = gf(e >> 1) * 2 + (e & 1);
+        const int te = L.tslot[(size_t)q * lnc + c];
+        if (te >= 0)
+          M.tslot[(size_t)q * nc + co[k] + c] = (te & 7) < 2 ? te : (((te >> 3) + bo[k]) << 3) | (te & 7);
+      }
+    }
+    for (int b = 0; b < L.ntiles; ++b) {
+      M.tile_cptr[to[k] + b + 1] = co[k] + L.tile_cptr[b + 1];
+      M.tptr[to[k] + b + 1] = qo[k] + L.tptr[b + 1];
+      tile_case[to[k] + b] = k;
+    }
+    case_tiles[k + 1] = to[k + 1];
+    for (int q = 0; q < L.tptr[L.ntiles]; ++q) M.tfaces[qo[k] + q] = io[k] + L.tfaces[q];
+    for (int f = 0; f < lnf; ++f) {
+      const int g = gf(f);
+      M.fl[g] = co[k] + L.fl[f];
+      M.fr[g] = f < lnif ? co[k] + L.fr[f] : -1;
+      M.fkind[g] = L.fkind[f];
+      M.fwall[g] = L.fwall[f];
+      for (int d = 0; d < 3; ++d) {
+        M.fn[(size_t)d * nf + g] = L.fn[(size_t)d * lnf + f];
+        M.fx[(size_t)d * nf + g] = L.fx[(size_t)d * lnf + f];
+      }
+      for (int d = 0; d < 10; ++d) M.ffr[(size_t)d * nf + g] = L.ffr[(size_t)d * lnf + f];
+      if (f >= lnif) bc_case[bo[k] + (f - lnif)] = k;
+    }
+    Layout().swap_out(L);  // release the part as soon as it is merged
+  }
+}
 
 ma_status plan_layout(const ma_mesh_data* md, Layout& L) {
   const int nc = md->num_owned_cells;
@@ -1035,37 +1204,6 @@ ma_status plan_layout(const ma_mesh_data* md, Layout& L) {
           g < nif ? (((side ? locr[g] : locl[g]) << 3) | side) : (((g - nif) << 3) | (3 + fkind[g]));
     }
 
-  // Every index a kernel will use, checked on the host before any upload.
-  {
-    std::vector<char> hit(nc, 0);
-    for (int c = 0; c < nc; ++c) {
-      if (dev_of[c] < 0 || dev_of[c] >= nc || hit[dev_of[c]]) return ma::fail(MA_ERR_INVALID, "layout: cell order");
-      hit[dev_of[c]] = 1;
-    }
-    if (tile_cptr.front() != 0 || tile_cptr.back() != nc) return ma::fail(MA_ERR_INVALID, "layout: tile cells");
-    for (int b = 0; b < ntiles; ++b) {
-      const int n = tile_cptr[b + 1] - tile_cptr[b];
-      if (n < 1 || n > kTile) return ma::fail(MA_ERR_INVALID, "layout: tile size");
-      for (int q = tptr[b]; q < tptr[b + 1]; ++q)
-        if (tfaces[q] < 0 || tfaces[q] >= nif) return ma::fail(MA_ERR_INVALID, "layout: tile face");
-    }
-    for (int f = 0; f < nf; ++f)
-      if (fl[f] < 0 || fl[f] >= nc || (f < nif && (fr[f] < 0 || fr[f] >= nc)))
-        return ma::fail(MA_ERR_INVALID, "layout: face cells");
-    for (int c = 0; c < nc; ++c) {
-      const int b = tile_of[c], nloc = tptr[b + 1] - tptr[b];
-      for (int q = 0; q < kMaxSlots; ++q) {
-        const int e = tslot[(size_t)q * nc + c], g = slot[(size_t)q * nc + c];
-        if ((e < 0) != (g < 0)) return ma::fail(MA_ERR_INVALID, "layout: slot tables");
-        if (e < 0) continue;
-        if ((g >> 1) >= nf) return ma::fail(MA_ERR_INVALID, "layout: slot face");
-        if ((e & 7) < 2 ? (e >> 3) >= nloc || tfaces[tptr[b] + (e >> 3)] != (g >> 1)
-                        : (e >> 3) >= nbf)
-          return ma::fail(MA_ERR_INVALID, "layout: tile slot");
-      }
-    }
-  }
-
   L.vol.assign(nc, 0.0);
   L.xc.assign((size_t)3 * nc, 0.0);
   for (int c = 0; c < nc; ++c) {
@@ -1078,10 +1216,114 @@ ma_status plan_layout(const ma_mesh_data* md, Layout& L) {
   return MA_OK;
 }
 
-ma_status build(ma_solver* s, const ma_mesh_data* md) {
+double effective_prandtl(const ma_options& o) { return o.prandtl > 0.0 ? o.prandtl : 0.71; }
+
+// Inflow state of a case: the explicit one, or TimeSolverExplicitRK4.h:218-223.
+Inflow case_inflow(const ma_options& o) {
+  const double ref_inflow[5] = {0.5805, 503.96, 0.0, 0.0, 343750.0};
+  Inflow in{};
+  for (int k = 0; k < 5; ++k) in.U[k] = o.use_inflow ? o.inflow[k] : ref_inflow[k];
+  return in;
+}
+
+ma_status check_options(const ma_options* opt) {
+  if (opt->cfl <= 0.0 && !(opt->dt > 0.0)) return ma::fail(MA_ERR_INVALID, "dt must be positive");
+  if (opt->limiter != MA_LIMITER_VENKAT && opt->limiter != MA_LIMITER_VANALBADA)
+    return ma::fail(MA_ERR_INVALID, "unknown limiter");
+  if (opt->precision != MA_PRECISION_FP64 && opt->precision != MA_PRECISION_FP32)
+    return ma::fail(MA_ERR_INVALID, "unknown precision");
+  return MA_OK;
+}
+
+ma_status check_mesh(const ma_mesh_data* md, const char* who) {
+  if (!md) return ma::fail(MA_ERR_INVALID, std::string(who) + ": null mesh");
+  if (md->num_ghosts != 0)
+    return ma::fail(MA_ERR_UNSUPPORTED, "ghost cells (multi-rank meshes) are not supported on one GPU");
+  if (md->num_owned_cells <= 0 || !md->cell_coordinates || !md->cell_volumes)
+    return ma::fail(MA_ERR_INVALID, std::string(who) + ": empty or null cell arrays");
+  if (md->num_boundary_groups < 0 || (md->num_boundary_groups > 0 && !md->boundary_faces))
+    return ma::fail(MA_ERR_INVALID, std::string(who) + ": bad boundary group list");
+  return MA_OK;
+}
+
+// Layout of a run over ncases meshes (concatenated when there are several),
+// checked, with the case of every tile, boundary face and device cell.
+struct RunPlan {
   Layout L;
-  ma_status st = plan_layout(md, L);
+  std::vector<int> tile_case, case_tiles, bc_case, cell_case, case_cell_off;
+};
+
+ma_status plan_run(int ncases, const ma_mesh_data* const* mds, RunPlan& P) {
+  P.case_cell_off.assign(ncases + 1, 0);
+  long long total_cells = 0, total_faces = 0;
+  for (int k = 0; k < ncases; ++k) {
+    ma_status st = check_mesh(mds[k], "plan");
+    if (st) return st;
+    total_cells += mds[k]->num_owned_cells;
+    total_faces += mds[k]->internal_faces.nfaces;
+    for (int g = 0; g < mds[k]->num_boundary_groups; ++g) total_faces += mds[k]->boundary_faces[g].faces.nfaces;
+    if (total_cells > (1ll << 28) || total_faces > (1ll << 28))
+      return ma::fail(MA_ERR_UNSUPPORTED, "more than 2^28 cells or faces in one run");
+    P.case_cell_off[k + 1] = (int)total_cells;
+  }
+  ma_status st = MA_OK;
+  Layout& L = P.L;
+  if (ncases == 1) {
+    if ((st = plan_layout(mds[0], L))) return st;
+    P.tile_case.assign(L.ntiles, 0);
+    P.case_tiles = {0, L.ntiles};
+    P.bc_case.assign(std::max(1, L.nbf), 0);
+    P.cell_case.assign(L.nc, 0);
+  } else {
+    // cases are planned independently on up to 16 host threads
+    std::vector<Layout> parts(ncases);
+    std::vector<ma_status> status(ncases, MA_OK);
+    std::vector<std::string> message(ncases);
+    std::atomic<int> next{0};
+    auto worker = [&] {
+      for (int k = next++; k < ncases; k = next++) {
+        try {
+          status[k] = plan_layout(mds[k], parts[k]);
+          if (status[k]) message[k] = ma::last_error();
+        } catch (const std::bad_alloc&) {
+          status[k] = MA_ERR_NOMEM;
+          message[k] = "plan: out of host memory";
+        }
+      }
+    };
+    const int nthreads = std::max(1, std::min({ncases, 16, (int)std::thread::hardware_concurrency()}));
+    std::vector<std::thread> pool;
+    for (int i = 1; i < nthreads; ++i) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+    for (int k = 0; k < ncases; ++k)
+      if (status[k]) return ma::fail(status[k], "case " + std::to_string(k) + ": " + message[k]);
+    merge_layouts(parts, L, P.tile_case, P.case_tiles, P.bc_case, P.cell_case);
+  }
+  return validate_layout(L);
+}
+
+void plan_stats(const Layout& L, long long* stats) {
+  stats[0] = L.dims[0];
+  stats[1] = L.dims[1];
+  stats[2] = L.dims[2];
+  stats[3] = L.ntiles;
+  stats[4] = L.max_faces;
+  stats[5] = L.tptr.empty() ? 0 : L.tptr.back();
+  stats[6] = L.derive_frame ? 1 : 0;
+}
+
+// Device arrays of one run: the layout, the per-case parameters and the work buffers.
+ma_status build(ma_solver* s, int ncases, const ma_mesh_data* const* mds) {
+  RunPlan P;
+  ma_status st = plan_run(ncases, mds, P);
   if (st) return st;
+  Layout& L = P.L;
+  std::vector<int>& tile_case = P.tile_case;
+  std::vector<int>& case_tiles = P.case_tiles;
+  std::vector<int>& bc_case = P.bc_case;
+  std::vector<int>& cell_case = P.cell_case;
+  s->case_cell_off.swap(P.case_cell_off);
   const int nc = L.nc, nif = L.nif, nf = L.nf;
   const long long nbf = L.nbf;
   s->dev_of.swap(L.dev_of);
@@ -1111,9 +1353,9 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
   m.fkind = dkind;
   m.fwall = dfw;
 
-  int *dtptr, *dtfaces, *dtslot, *dtcptr;
+  int *dtptr, *dtfaces, *dtslot, *dtcptr, *dtcase;
   if ((st = upload(s, &dtptr, L.tptr)) || (st = upload(s, &dtfaces, L.tfaces)) || (st = upload(s, &dtslot, L.tslot)) ||
-      (st = upload(s, &dtcptr, L.tile_cptr)))
+      (st = upload(s, &dtcptr, L.tile_cptr)) || (st = upload(s, &dtcase, tile_case)))
     return st;
   s->t.ntiles = L.ntiles;
   s->t.per_xcd = (L.ntiles + kXcds - 1) / kXcds;
@@ -1122,6 +1364,38 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
   s->t.faces = dtfaces;
   s->t.slot = dtslot;
   s->t.cptr = dtcptr;
+  s->t.tcase = dtcase;
+  L = Layout();
+
+  // per-case parameters
+  std::vector<double> dt0(ncases), cfl(ncases), zero(ncases, 0.0);
+  std::vector<IcParams> ic(ncases);
+  std::vector<Inflow> inflow(ncases);
+  s->any_cfl = false;
+  for (int k = 0; k < ncases; ++k) {
+    const ma_options& o = s->case_opt[k];
+    dt0[k] = o.dt;
+    cfl[k] = o.cfl > 0.0 ? o.cfl : 0.0;
+    s->any_cfl = s->any_cfl || o.cfl > 0.0;
+    inflow[k] = case_inflow(o);
+    IcParams& p = ic[k];
+    p = IcParams{};
+    p.kind = o.ic_kind;
+    p.problem_type = o.problem_type;
+    p.midx = o.lx / 2.0;
+    for (int q = 0; q < 5; ++q) {
+      p.ic[q] = o.ic[q];
+      p.inflow[q] = inflow[k].U[q];
+    }
+    p.perturb = o.perturb;
+    p.seed = o.seed;
+    p.cell0 = s->case_cell_off[k];
+  }
+  if ((st = upload(s, &s->dt_dev, dt0)) || (st = upload(s, &s->cfl_dev, cfl)) ||
+      (st = upload(s, &s->time_dev, zero)) || (st = upload(s, &s->ic_dev, ic)) ||
+      (st = upload(s, &s->inflow_dev, inflow)) || (st = upload(s, &s->cell_case, cell_case)) ||
+      (st = upload(s, &s->bc_case, bc_case)) || (st = upload(s, &s->case_tiles, case_tiles)))
+    return st;
 
   const size_t n5 = (size_t)5 * nc;
   const size_t tsize = s->fp32 ? sizeof(float) : sizeof(double);
@@ -1129,14 +1403,11 @@ ma_status build(ma_solver* s, const ma_mesh_data* md) {
       (st = s->alloc_bytes(&s->V, n5 * tsize)) || (st = s->alloc_bytes(&s->grad, 3 * n5 * tsize)) ||
       (st = s->alloc_bytes(&s->phi, n5 * tsize)) ||
       (st = s->alloc_bytes(&s->bc, (size_t)5 * std::max<long long>(nbf, 1) * tsize)) ||
-      (st = s->alloc(&s->dt_dev, 1)) || (st = s->alloc(&s->time_dev, 1)) ||
-      (st = s->alloc(&s->partial, (size_t)nblocks(nc))) || (st = s->alloc(&s->counter, 1)))
+      (st = s->alloc(&s->partial, (size_t)std::max(1, s->t.ntiles))) || (st = s->alloc(&s->counter, 1)))
     return st;
   MA_HIP(hipMemset(s->R, 0, n5 * sizeof(double)));
   MA_HIP(hipMemset(s->grad, 0, 3 * n5 * tsize));
   MA_HIP(hipMemset(s->phi, 0, n5 * tsize));
-  MA_HIP(hipMemcpy(s->dt_dev, &s->opt.dt, sizeof(double), hipMemcpyHostToDevice));
-  MA_HIP(hipMemset(s->time_dev, 0, sizeof(double)));
   return MA_OK;
 }
 
@@ -1161,14 +1432,15 @@ void run_stage(ma_solver* s, int irk) {
                          per);
     });
   }
-  Visc visc{s->opt.prandtl > 0.0 ? s->opt.prandtl : 0.71};
+  Visc visc{effective_prandtl(s->opt)};
   // Stage 0 accumulates from sol_n (sol_np1 == sol_n at the start of a step).
   const double* acc_in = irk == 0 ? Un : Uacc;
   const int nbf = m.nf - m.nif;
   if (nbf > 0) {
     s->launch("k_bc", [&] {
       hipLaunchKernelGGL(k_bc<T>, dim3(nblocks(nbf)), dim3(kBlock), 0, s->stream, m,
-                         static_cast<const T*>(s->V), visc.prandtl, s->inflow, static_cast<T*>(s->bc));
+                         static_cast<const T*>(s->V), visc.prandtl, s->inflow_dev, s->bc_case,
+                         static_cast<T*>(s->bc));
     });
   }
   const size_t lds = (size_t)s->t.max_faces * 5 * sizeof(T);
@@ -1183,22 +1455,20 @@ void run_stage(ma_solver* s, int irk) {
 template <bool SO, class Visc, class Lim, class T>
 void run_steps(ma_solver* s, int nsteps) {
   for (int it = 0; it < nsteps; ++it) {
-    if (s->opt.cfl > 0.0) {
-      const int nb = nblocks(s->m.nc);
+    // per-case dt (CFL minimum or fixed) and time += dt
+    if (s->any_cfl) {
       s->launch("k_dt_partial", [&] {
-        hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(kBlock), 0, s->stream, s->m, s->U[s->cur],
-                           s->opt.viscous, s->opt.prandtl > 0.0 ? s->opt.prandtl : 0.71, s->partial);
+        hipLaunchKernelGGL(k_dt_partial, dim3(s->t.ntiles), dim3(kTile), 0, s->stream, s->m, s->t,
+                           s->U[s->cur], s->opt.viscous, effective_prandtl(s->opt), s->partial);
       });
-      s->launch("k_dt_final", [&] {
-        hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kBlock), 0, s->stream, s->partial, nb, s->opt.cfl,
-                           s->dt_dev, s->time_dev);
-      });
-    } else {
-      s->time += s->opt.dt;
-      s->last_dt = s->opt.dt;
     }
+    s->launch("k_dt_final", [&] {
+      hipLaunchKernelGGL(k_dt_final, dim3(s->ncases), dim3(kBlock), 0, s->stream, s->partial, s->case_tiles,
+                         s->cfl_dev, s->dt_dev, s->time_dev);
+    });
     for (int irk = 0; irk < 4; ++irk) run_stage<SO, Visc, Lim, T>(s, irk);
     s->cur = 1 - s->cur;  // U_acc after the last stage is the new sol_n
+    ++s->steps_done;
   }
 }
 
@@ -1243,66 +1513,58 @@ ma_status download_aos(ma_solver* s, const S* dsrc, int ncomp, double* host) {
 
 extern "C" const char* ma_last_error(void) { return ma::last_error().c_str(); }
 
-extern "C" ma_status ma_mesh_plan(const ma_mesh_data* md, long long* stats) {
-  if (!md || !stats) return ma::fail(MA_ERR_INVALID, "ma_mesh_plan: null argument");
-  if (md->num_ghosts != 0)
-    return ma::fail(MA_ERR_UNSUPPORTED, "ghost cells (multi-rank meshes) are not supported on one GPU");
-  if (md->num_owned_cells <= 0 || !md->cell_coordinates || !md->cell_volumes)
-    return ma::fail(MA_ERR_INVALID, "ma_mesh_plan: empty or null cell arrays");
-  if (md->num_boundary_groups < 0 || (md->num_boundary_groups > 0 && !md->boundary_faces))
-    return ma::fail(MA_ERR_INVALID, "ma_mesh_plan: bad boundary group list");
-  Layout L;
+extern "C" ma_status ma_batch_plan(int ncases, const ma_mesh_data* const* meshes, long long* stats) {
+  if (!meshes || !stats || ncases < 1) return ma::fail(MA_ERR_INVALID, "plan: bad argument");
+  RunPlan P;
   ma_status st = MA_OK;
   try {
-    st = plan_layout(md, L);
+    st = plan_run(ncases, meshes, P);
   } catch (const std::bad_alloc&) {
-    st = ma::fail(MA_ERR_NOMEM, "ma_mesh_plan: out of host memory");
+    st = ma::fail(MA_ERR_NOMEM, "plan: out of host memory");
   }
   if (st) return st;
-  stats[0] = L.dims[0];
-  stats[1] = L.dims[1];
-  stats[2] = L.dims[2];
-  stats[3] = L.ntiles;
-  stats[4] = L.max_faces;
-  stats[5] = L.tptr.empty() ? 0 : L.tptr.back();
-  stats[6] = L.derive_frame ? 1 : 0;
+  plan_stats(P.L, stats);
   return MA_OK;
 }
 
-extern "C" ma_status ma_solver_create(const ma_mesh_data* md, const ma_options* opt, ma_solver** out) {
-  if (!md || !opt || !out) return ma::fail(MA_ERR_INVALID, "ma_solver_create: null argument");
-  *out = nullptr;
-  if (md->num_ghosts != 0)
-    return ma::fail(MA_ERR_UNSUPPORTED, "ghost cells (multi-rank meshes) are not supported on one GPU");
-  if (md->num_owned_cells <= 0 || !md->cell_coordinates || !md->cell_volumes)
-    return ma::fail(MA_ERR_INVALID, "ma_solver_create: empty or null cell arrays");
-  if (md->num_boundary_groups < 0 || (md->num_boundary_groups > 0 && !md->boundary_faces))
-    return ma::fail(MA_ERR_INVALID, "ma_solver_create: bad boundary group list");
-  if (opt->cfl <= 0.0 && !(opt->dt > 0.0)) return ma::fail(MA_ERR_INVALID, "dt must be positive");
-  if (opt->limiter != MA_LIMITER_VENKAT && opt->limiter != MA_LIMITER_VANALBADA)
-    return ma::fail(MA_ERR_INVALID, "unknown limiter");
-  if (opt->precision != MA_PRECISION_FP64 && opt->precision != MA_PRECISION_FP32)
-    return ma::fail(MA_ERR_INVALID, "unknown precision");
+extern "C" ma_status ma_mesh_plan(const ma_mesh_data* md, long long* stats) {
+  if (!md) return ma::fail(MA_ERR_INVALID, "ma_mesh_plan: null argument");
+  return ma_batch_plan(1, &md, stats);
+}
+
+namespace {
+
+// Shared body of ma_solver_create and ma_batch_create.
+ma_status create_run(int ncases, const ma_mesh_data* const* mds, const ma_options* opts, ma_solver** out) {
+  const ma_options& o0 = opts[0];
+  for (int k = 0; k < ncases; ++k) {
+    ma_status st = check_mesh(mds[k], "create");
+    if (st || (st = check_options(&opts[k]))) return st;
+    const ma_options& o = opts[k];
+    if (o.second_order_space != o0.second_order_space || o.viscous != o0.viscous ||
+        o.limiter != o0.limiter || o.precision != o0.precision ||
+        effective_prandtl(o) != effective_prandtl(o0))
+      return ma::fail(MA_ERR_INVALID,
+                      "batched cases must share second_order_space, viscous, limiter, precision and prandtl");
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return ma::fail(MA_ERR_DEVICE, "no HIP device available");
   ma_solver* s = new (std::nothrow) ma_solver();
-  if (!s) return ma::fail(MA_ERR_NOMEM, "ma_solver_create: out of host memory");
-  s->opt = *opt;
-  s->fp32 = opt->precision == MA_PRECISION_FP32;
-  // TimeSolverExplicitRK4.h:218-223
-  const double ref_inflow[5] = {0.5805, 503.96, 0.0, 0.0, 343750.0};
-  for (int k = 0; k < 5; ++k) s->inflow.U[k] = opt->use_inflow ? opt->inflow[k] : ref_inflow[k];
-  (void)hipGetDevice(&s->device);
+  if (!s) return ma::fail(MA_ERR_NOMEM, "create: out of host memory");
   ma_status st = MA_OK;
-  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete s;
-    return ma::fail(MA_ERR_DEVICE, "hipStreamCreate failed");
-  }
   try {
-    st = build(s, md);
+    s->opt = o0;
+    s->ncases = ncases;
+    s->case_opt.assign(opts, opts + ncases);
+    s->fp32 = o0.precision == MA_PRECISION_FP32;
+    (void)hipGetDevice(&s->device);
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+      st = ma::fail(MA_ERR_DEVICE, "hipStreamCreate failed");
+    else
+      st = build(s, ncases, mds);
   } catch (const std::bad_alloc&) {
-    st = ma::fail(MA_ERR_NOMEM, "ma_solver_create: out of host memory");
+    st = ma::fail(MA_ERR_NOMEM, "create: out of host memory");
   }
   if (st) {
     ma_solver_destroy(s);
@@ -1312,27 +1574,32 @@ extern "C" ma_status ma_solver_create(const ma_mesh_data* md, const ma_options* 
   return MA_OK;
 }
 
+}  // namespace
+
+extern "C" ma_status ma_solver_create(const ma_mesh_data* md, const ma_options* opt, ma_solver** out) {
+  if (!md || !opt || !out) return ma::fail(MA_ERR_INVALID, "ma_solver_create: null argument");
+  *out = nullptr;
+  return create_run(1, &md, opt, out);
+}
+
+extern "C" ma_status ma_batch_create(int ncases, const ma_mesh_data* const* meshes, const ma_options* opts,
+                                     ma_solver** out) {
+  if (!meshes || !opts || !out || ncases < 1) return ma::fail(MA_ERR_INVALID, "ma_batch_create: bad argument");
+  *out = nullptr;
+  return create_run(ncases, meshes, opts, out);
+}
+
 extern "C" ma_status ma_solver_initialize(ma_solver* s) {
   if (!s) return ma::fail(MA_ERR_INVALID, "null solver");
-  IcParams p{};
-  p.kind = s->opt.ic_kind;
-  p.problem_type = s->opt.problem_type;
-  p.midx = s->opt.lx / 2.0;
-  for (int k = 0; k < 5; ++k) {
-    p.ic[k] = s->opt.ic[k];
-    p.inflow[k] = s->inflow.U[k];
-  }
-  p.perturb = s->opt.perturb;
-  p.seed = s->opt.seed;
   s->cur = 0;
-  hipLaunchKernelGGL(k_init, dim3(nblocks(s->m.nc)), dim3(kBlock), 0, s->stream, s->m, p, s->U[0]);
+  hipLaunchKernelGGL(k_init, dim3(nblocks(s->m.nc)), dim3(kBlock), 0, s->stream, s->m, s->ic_dev, s->cell_case,
+                     s->U[0]);
   MA_HIP(hipGetLastError());
   const size_t n5 = (size_t)5 * s->m.nc;
   MA_HIP(hipMemsetAsync(s->R, 0, n5 * sizeof(double), s->stream));
-  MA_HIP(hipMemsetAsync(s->time_dev, 0, sizeof(double), s->stream));
+  MA_HIP(hipMemsetAsync(s->time_dev, 0, s->ncases * sizeof(double), s->stream));
   MA_HIP(hipStreamSynchronize(s->stream));
-  s->time = 0.0;
-  s->last_dt = 0.0;
+  s->steps_done = 0;
   return MA_OK;
 }
 
@@ -1386,16 +1653,55 @@ extern "C" ma_status ma_solver_get_gradients(ma_solver* s, double* grad, double*
   return MA_OK;
 }
 
+namespace {
+
+ma_status case_time(ma_solver* s, int k, double* time, double* last_dt) {
+  double t = 0.0, d = 0.0;
+  MA_HIP(hipMemcpyAsync(&t, s->time_dev + k, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  MA_HIP(hipMemcpyAsync(&d, s->dt_dev + k, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  MA_HIP(hipStreamSynchronize(s->stream));
+  if (time) *time = t;
+  if (last_dt) *last_dt = s->steps_done ? d : 0.0;
+  return MA_OK;
+}
+
+}  // namespace
+
 extern "C" ma_status ma_solver_get_time(ma_solver* s, double* time, double* last_dt) {
   if (!s) return ma::fail(MA_ERR_INVALID, "null solver");
-  double t = s->time, d = s->last_dt;
-  if (s->opt.cfl > 0.0) {
-    MA_HIP(hipMemcpyAsync(&t, s->time_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    MA_HIP(hipMemcpyAsync(&d, s->dt_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    MA_HIP(hipStreamSynchronize(s->stream));
+  return case_time(s, 0, time, last_dt);
+}
+
+extern "C" ma_status ma_batch_get_case_time(ma_solver* s, int k, double* time, double* last_dt) {
+  if (!s || k < 0 || k >= s->ncases) return ma::fail(MA_ERR_INVALID, "ma_batch_get_case_time: bad argument");
+  return case_time(s, k, time, last_dt);
+}
+
+extern "C" ma_status ma_batch_get_case_state(ma_solver* s, int k, double* U) {
+  if (!s || !U || k < 0 || k >= s->ncases) return ma::fail(MA_ERR_INVALID, "ma_batch_get_case_state: bad argument");
+  const int nc = s->m.nc, c0 = s->case_cell_off[k], c1 = s->case_cell_off[k + 1];
+  // the case's device cells are one contiguous range
+  int lo = nc, hi = -1;
+  for (int c = c0; c < c1; ++c) {
+    lo = std::min(lo, s->dev_of[c]);
+    hi = std::max(hi, s->dev_of[c]);
   }
-  if (time) *time = t;
-  if (last_dt) *last_dt = d;
+  const int n = hi - lo + 1;
+  std::vector<double> tmp((size_t)5 * n);
+  for (int q = 0; q < 5; ++q)
+    MA_HIP(hipMemcpyAsync(tmp.data() + (size_t)q * n, s->U[s->cur] + (size_t)q * nc + lo, n * sizeof(double),
+                          hipMemcpyDeviceToHost, s->stream));
+  MA_HIP(hipStreamSynchronize(s->stream));
+  for (int c = c0; c < c1; ++c)
+    for (int q = 0; q < 5; ++q) U[(size_t)(c - c0) * 5 + q] = tmp[(size_t)q * n + s->dev_of[c] - lo];
+  return MA_OK;
+}
+
+extern "C" ma_status ma_batch_sizes(ma_solver* s, int* ncases, int* case_cell_offsets) {
+  if (!s || !ncases) return ma::fail(MA_ERR_INVALID, "ma_batch_sizes: null argument");
+  *ncases = s->ncases;
+  if (case_cell_offsets)
+    for (int k = 0; k <= s->ncases; ++k) case_cell_offsets[k] = s->case_cell_off[k];
   return MA_OK;
 }
 

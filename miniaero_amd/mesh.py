@@ -33,25 +33,28 @@ class Faces:
         return int(self.face_cell_conn.shape[0])
 
     def _c(self) -> _lib.ma_faces:
+        """C view of the arrays; the arrays it points to are kept alive in the
+        returned object's `_keep` list."""
         f = _lib.ma_faces()
         f.nfaces = self.nfaces_
         D, I = _lib._D, _lib._I
-        self._keep = [np.ascontiguousarray(self.coordinates, np.float64),
-                      np.ascontiguousarray(self.face_normal, np.float64),
-                      np.ascontiguousarray(self.face_tangent, np.float64),
-                      np.ascontiguousarray(self.face_binormal, np.float64),
-                      np.ascontiguousarray(self.face_cell_conn, np.int32)]
+        keep = [np.ascontiguousarray(self.coordinates, np.float64),
+                np.ascontiguousarray(self.face_normal, np.float64),
+                np.ascontiguousarray(self.face_tangent, np.float64),
+                np.ascontiguousarray(self.face_binormal, np.float64),
+                np.ascontiguousarray(self.face_cell_conn, np.int32)]
         f.coordinates, f.face_normal, f.face_tangent, f.face_binormal = [
-            a.ctypes.data_as(D) for a in self._keep[:4]]
-        f.face_cell_conn = self._keep[4].ctypes.data_as(I)
+            a.ctypes.data_as(D) for a in keep[:4]]
+        f.face_cell_conn = keep[4].ctypes.data_as(I)
         if self.cell_flux_index is not None:
             a = np.ascontiguousarray(self.cell_flux_index, np.int32)
-            self._keep.append(a)
+            keep.append(a)
             f.cell_flux_index = a.ctypes.data_as(I)
         if self.permute_vector is not None:
             a = np.ascontiguousarray(self.permute_vector, np.int32)
-            self._keep.append(a)
+            keep.append(a)
             f.permute_vector = a.ctypes.data_as(I)
+        f._keep = keep
         return f
 
 
@@ -68,25 +71,34 @@ class MeshData:
     def plan(self) -> dict:
         """Host-only pre-flight check (ma_mesh_plan): validates the arrays and
         reports the device layout the solver would build."""
+        c = self.to_c()
         st = (ctypes.c_longlong * 7)()
-        _lib.check(_lib.load().ma_mesh_plan(ctypes.byref(self.to_c()), st))
-        return dict(lattice=(st[0], st[1], st[2]), tiles=st[3], max_tile_faces=st[4],
-                    tile_face_entries=st[5], derived_frames=bool(st[6]))
+        _lib.check(_lib.load().ma_mesh_plan(ctypes.byref(c), st))
+        return _plan_dict(st)
+
 
     def to_c(self) -> _lib.ma_mesh_data:
+        """C view of the mesh; every array it points to is kept alive in the
+        returned object's `_keep` list, so the view stays valid as long as
+        the returned object does."""
         m = _lib.ma_mesh_data()
         m.num_owned_cells = int(self.num_owned_cells)
         m.num_ghosts = int(self.num_ghosts)
-        self._keep = [np.ascontiguousarray(self.cell_coordinates, np.float64),
-                      np.ascontiguousarray(self.cell_volumes, np.float64)]
-        m.cell_coordinates = self._keep[0].ctypes.data_as(_lib._D)
-        m.cell_volumes = self._keep[1].ctypes.data_as(_lib._D)
-        m.internal_faces = self.internal_faces._c()
+        keep = [np.ascontiguousarray(self.cell_coordinates, np.float64),
+                np.ascontiguousarray(self.cell_volumes, np.float64)]
+        m.cell_coordinates = keep[0].ctypes.data_as(_lib._D)
+        m.cell_volumes = keep[1].ctypes.data_as(_lib._D)
+        fi = self.internal_faces._c()
+        keep.append(fi._keep)
+        m.internal_faces = fi
         groups = (_lib.ma_bc_group * max(1, len(self.boundary_faces)))()
         for g, (name, faces) in enumerate(self.boundary_faces):
             groups[g].kind = _lib.BC_KINDS[name]
-            groups[g].faces = faces._c()
-        self._groups = groups
+            fg = faces._c()
+            keep.append(fg._keep)
+            groups[g].faces = fg
+        keep.append(groups)
+        m._keep = keep
         m.num_boundary_groups = len(self.boundary_faces)
         m.boundary_faces = ctypes.cast(groups, ctypes.POINTER(_lib.ma_bc_group))
         return m
@@ -151,3 +163,34 @@ class Parallel3DMesh:
                             for g in range(v.num_boundary_groups)])
         md._owner = owner  # the arrays above are views into the generated mesh
         return md
+
+
+def _plan_dict(st) -> dict:
+    return dict(lattice=(st[0], st[1], st[2]), tiles=st[3], max_tile_faces=st[4],
+                tile_face_entries=st[5], derived_frames=bool(st[6]))
+
+
+def plan_batch(meshes) -> dict:
+    """ma_batch_plan: the checked layout of several meshes stepped together."""
+    cs = [md.to_c() for md in meshes]
+    ptrs = (ctypes.POINTER(_lib.ma_mesh_data) * len(cs))(*[ctypes.pointer(c) for c in cs])
+    st = (ctypes.c_longlong * 7)()
+    _lib.check(_lib.load().ma_batch_plan(len(cs), ptrs, st))
+    return _plan_dict(st)
+
+
+def generate_meshes(options_list, max_workers: int | None = None) -> list:
+    """Parallel3DMesh(...).fillMeshData() for every Options of the list, on
+    up to max_workers threads (the generator is thread-safe and runs outside
+    the interpreter lock); results in list order."""
+    from concurrent.futures import ThreadPoolExecutor
+    import os
+
+    def one(o):
+        return Parallel3DMesh(o.nx, o.ny, o.nz, o.lx, o.ly, o.lz, o.problem_type, o.angle,
+                              options=o).fillMeshData()
+    workers = max_workers or min(16, os.cpu_count() or 1)
+    if workers <= 1 or len(options_list) <= 1:
+        return [one(o) for o in options_list]
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(one, options_list))

@@ -130,3 +130,47 @@ class TimeSolverExplicitRK4:
             self.close()
         except Exception:
             pass
+
+
+class BatchSolver(TimeSolverExplicitRK4):
+    """Independent cases stepped together in one set of launches
+    (ma_batch_create).  The reference runs one mesh per process (Main.C:96-141);
+    here each case keeps its own mesh, initial state, inflow and dt or CFL,
+    and its result equals a TimeSolverExplicitRK4 run on it alone bit for bit.
+    state()/residual()/gradients() cover all cases' cells in order."""
+
+    def __init__(self, cases):
+        cases = list(cases)
+        if not cases:
+            raise ValueError("BatchSolver needs at least one case")
+        self.cases = cases
+        self.mesh_data = cases[0][0]
+        self.options = cases[0][1]
+        self._L = _lib.load()
+        self._c_meshes = [md.to_c() for md, _ in cases]
+        n = len(cases)
+        ptrs = (ctypes.POINTER(_lib.ma_mesh_data) * n)(*[ctypes.pointer(m) for m in self._c_meshes])
+        opts = (_lib.ma_options * n)(*[o.to_c() for _, o in cases])
+        self._keep = (ptrs, opts)
+        h = ctypes.c_void_p()
+        _lib.check(self._L.ma_batch_create(n, ptrs, opts, ctypes.byref(h)))
+        self._h = h
+        self.ncases = n
+        off = (ctypes.c_int * (n + 1))()
+        nn = ctypes.c_int()
+        _lib.check(self._L.ma_batch_sizes(self._h, ctypes.byref(nn), off))
+        self.offsets = np.array(off[:], dtype=np.int64)
+        self.ncells = int(self.offsets[-1])
+
+    def Solve(self, *a, **kw):
+        raise NotImplementedError("BatchSolver: step the cases with initialize()/step()")
+
+    def case_state(self, k: int) -> np.ndarray:
+        U = np.empty((int(self.offsets[k + 1] - self.offsets[k]), 5))
+        _lib.check(self._L.ma_batch_get_case_state(self._h, int(k), U.ctypes.data_as(_lib._D)))
+        return U
+
+    def case_time(self, k: int):
+        t, d = ctypes.c_double(), ctypes.c_double()
+        _lib.check(self._L.ma_batch_get_case_time(self._h, int(k), ctypes.byref(t), ctypes.byref(d)))
+        return t.value, d.value

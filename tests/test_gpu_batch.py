@@ -1,0 +1,119 @@
+"""Batched form: independent cases stepped together in one set of launches.
+
+Bar: every case of a batch equals, bit for bit, the same case run alone
+(which the parity tests hold bit for bit against the oracle), and its time
+and last dt are the same.  Cases differ in mesh size and geometry, initial
+state, inflow, fixed dt or CFL number and seed.
+"""
+import numpy as np
+import pytest
+
+import miniaero_amd as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+EXT = ("bc_kind", "y_stretch", "limiter", "ic_kind", "ic", "use_inflow", "inflow", "prandtl",
+       "cfl", "perturb", "seed")
+
+
+def make_case(kw):
+    opts = M.Options(**{k: v for k, v in kw.items() if k in M.Options.__dataclass_fields__})
+    ext = {k: v for k, v in kw.items() if k in EXT}
+    md = M.Parallel3DMesh(kw["nx"], kw["ny"], kw["nz"], kw["lx"], kw["ly"], kw["lz"],
+                          kw["problem_type"], kw.get("angle", 0.0),
+                          options=M.Options(**ext)).fillMeshData()
+    return md, opts
+
+
+def plate(**kw):
+    base = dict(problem_type=1, lx=1.0, ly=0.05, lz=0.025, nx=16, ny=12, nz=4, dt=1e-7,
+                second_order_space=1, viscous=1)
+    base.update(kw)
+    return base
+
+
+VISCOUS_CASES = [
+    plate(nx=16, ny=32, nz=2, lx=2.0, ly=0.002, lz=1.0, dt=3e-8),
+    plate(nx=24, ny=16, nz=4, dt=0.0, y_stretch=1.05, bc_kind=(0, 2, 0, 0, 0, 0), use_inflow=1,
+          inflow=(0.01772, 0.01772 * 104.2, 0.0, 0.0,
+                  0.01772 * 287.05 / 0.4 * 300.0 + 0.5 * 0.01772 * 104.2 ** 2), cfl=0.5),
+    plate(nx=12, ny=10, nz=6, ic_kind=2, perturb=0.01, seed=3),
+    dict(problem_type=2, lx=1.0, ly=0.5, lz=0.5, angle=10.0, nx=1, ny=1, nz=1, dt=1e-7,
+         second_order_space=1, viscous=1),
+    plate(nx=12, ny=10, nz=6, ic_kind=2, perturb=0.02, seed=11, cfl=0.3, dt=0.0),
+    dict(problem_type=0, lx=1.0, ly=0.25, lz=0.25, nx=32, ny=8, nz=8, dt=1e-5,
+         second_order_space=1, viscous=1, ic_kind=1, ic=(1.0, 1.0, 0.125, 0.1, 0.4),
+         bc_kind=(1, 1, 1, 1, 1, 1)),
+]
+
+VANALBADA_CASES = [
+    dict(problem_type=0, lx=1.0, ly=0.25, lz=0.25, nx=nx, ny=ny, nz=nz, dt=dt,
+         second_order_space=1, viscous=0, limiter=1, ic_kind=1, ic=ic, bc_kind=(1, 1, 1, 1, 1, 1))
+    for nx, ny, nz, dt, ic in (
+        (32, 8, 8, 1e-4, (1.0, 1.0, 0.125, 0.1, 0.5)),
+        (17, 5, 3, 2e-4, (2.0, 3.0, 0.5, 0.4, 0.3)),
+        (40, 4, 6, 5e-5, (1.0, 1.0, 0.125, 0.1, 0.7)),
+    )
+] + [dict(problem_type=2, lx=2.0, ly=2.0, lz=1.0, angle=30.0, nx=16, ny=8, nz=3, dt=1e-5,
+          second_order_space=1, viscous=0, limiter=1)]
+
+
+def run_batch_vs_alone(kws, nsteps):
+    cases = [make_case(kw) for kw in kws]
+    print(f"[gpu-batch] {len(cases)} cases, cells "
+          f"{[int(md.num_owned_cells) for md, _ in cases]}, {nsteps} steps", flush=True)
+    b = M.BatchSolver(cases)
+    try:
+        b.initialize()
+        b.step(nsteps)
+        got = [b.case_state(k) for k in range(len(cases))]
+        times = [b.case_time(k) for k in range(len(cases))]
+        whole = b.state()
+    finally:
+        b.close()
+    assert np.array_equal(whole, np.vstack(got))
+    for k, (md, opts) in enumerate(cases):
+        s = M.TimeSolverExplicitRK4(md, opts)
+        try:
+            s.initialize()
+            s.step(nsteps)
+            U = s.state()
+            t = s.time()
+        finally:
+            s.close()
+        assert np.array_equal(got[k], U), f"case {k} differs from its own run"
+        assert times[k] == t, f"case {k}: time {times[k]} vs {t}"
+    return got
+
+
+def test_viscous_batch_matches_cases_alone():
+    run_batch_vs_alone(VISCOUS_CASES, 12)
+
+
+def test_vanalbada_batch_matches_cases_alone():
+    run_batch_vs_alone(VANALBADA_CASES, 15)
+
+
+def test_batch_case_matches_oracle():
+    """One batched case against the oracle directly (not only via its own run)."""
+    kw = VISCOUS_CASES[2]
+    got = run_batch_vs_alone([VISCOUS_CASES[0], kw, VISCOUS_CASES[3]], 5)
+    ok = O.make_options(**{k: v for k, v in kw.items()})
+    om = O.OracleMesh(ok)
+    cpu = O.OracleSolver(om)
+    cpu.init()
+    cpu.step(5)
+    assert np.array_equal(got[1], cpu.state())
+
+
+def test_many_small_cases():
+    rng = np.random.default_rng(5)
+    kws = []
+    for i in range(40):
+        nx, ny, nz = (int(x) for x in rng.integers(1, 9, size=3))
+        kws.append(dict(problem_type=0, lx=1.0, ly=0.5, lz=0.5, nx=nx, ny=ny, nz=nz,
+                        dt=float(rng.uniform(1e-5, 1e-4)), second_order_space=1, viscous=0, limiter=1,
+                        ic_kind=1, ic=(1.0, 1.0, 0.125, 0.1, float(rng.uniform(0.2, 0.8))),
+                        bc_kind=(1, 1, 1, 1, 1, 1)))
+    run_batch_vs_alone(kws, 6)

@@ -177,3 +177,67 @@ def test_layout_plan_detects_derivable_face_frames():
                      M.Faces(F.coordinates, F.face_normal, t, F.face_binormal, F.face_cell_conn),
                      md.boundary_faces)
     assert not bad.plan()["derived_frames"]
+
+
+def test_batch_plan_concatenates_case_layouts():
+    """The batched layout is the cases' layouts side by side: tiles and tile
+    face entries add up, the largest tile is the largest of any case, and
+    the merged layout passes the library's own consistency check (every cell
+    in one tile, every slot pointing at a face of that cell)."""
+    from miniaero_amd.mesh import plan_batch
+    meshes = [M.Parallel3DMesh(9, 7, 5, 1.0, 0.5, 0.25, 1, options=M.Options(y_stretch=1.05)).fillMeshData(),
+              M.Parallel3DMesh(16, 8, 3, 2.0, 2.0, 1.0, 2, 30.0).fillMeshData(),
+              M.Parallel3DMesh(1, 1, 1, 1.0, 1.0, 1.0, 0).fillMeshData()]
+    # a mesh that is not a lattice: one cell relabelled and a face pair swapped
+    md = M.Parallel3DMesh(4, 4, 4, 1.0, 1.0, 1.0, 0).fillMeshData()
+    rng = np.random.default_rng(2)
+    nc = md.num_owned_cells
+    new = rng.permutation(nc)
+    old = np.argsort(new)
+    def relabel(F):
+        conn = F.face_cell_conn.copy()
+        m = conn >= 0
+        conn[m] = new[conn[m]]
+        return M.Faces(F.coordinates, F.face_normal, F.face_tangent, F.face_binormal, conn)
+    meshes.append(M.MeshData(nc, md.cell_coordinates[old], md.cell_volumes[old],
+                             relabel(md.internal_faces), [(n, relabel(F)) for n, F in md.boundary_faces]))
+    parts = [m.plan() for m in meshes]
+    p = plan_batch(meshes)
+    assert p["lattice"] == (0, 0, 0)
+    assert p["tiles"] == sum(q["tiles"] for q in parts)
+    assert p["tile_face_entries"] == sum(q["tile_face_entries"] for q in parts)
+    assert p["max_tile_faces"] == max(q["max_tile_faces"] for q in parts)
+    assert p["derived_frames"] == all(q["derived_frames"] for q in parts)
+    assert plan_batch(meshes[:1]) == parts[0]
+
+
+def test_batch_create_rejects_mixed_schemes():
+    md = M.Parallel3DMesh(2, 2, 2, 1, 1, 1, 0).fillMeshData()
+    a = M.Options(dt=1e-6, second_order_space=1)
+    for b in (a.replace(second_order_space=0), a.replace(viscous=1), a.replace(limiter=1),
+              a.replace(precision=1), a.replace(prandtl=0.72)):
+        with pytest.raises(M.MiniAeroError) as e:
+            M.BatchSolver([(md, a), (md, b)])
+        assert e.value.status == _lib.MA_ERR_INVALID
+        assert "batched cases must share" in str(e.value)
+    with pytest.raises(M.MiniAeroError) as e:
+        M.BatchSolver([(md, a), (md, a.replace(dt=0.0))])
+    assert e.value.status == _lib.MA_ERR_INVALID
+
+
+def test_mesh_generation_on_threads_is_bit_exact():
+    """Meshes generated concurrently equal meshes generated one at a time
+    (the face shuffle draws from a private copy of rand()'s generator)."""
+    from miniaero_amd.mesh import generate_meshes
+    opts = [M.Options(problem_type=p, nx=nx, ny=ny, nz=nz, lx=1.0, ly=0.5, lz=0.5,
+                      angle=10.0 if p == 2 else 0.0)
+            for p, nx, ny, nz in ((0, 9, 7, 5), (2, 12, 6, 3), (1, 5, 11, 4), (0, 16, 4, 4))] * 3
+    serial = generate_meshes(opts, max_workers=1)
+    threaded = generate_meshes(opts, max_workers=8)
+    for a, b in zip(serial, threaded):
+        for Fa, Fb in [(a.internal_faces, b.internal_faces)] + [(x[1], y[1]) for x, y in
+                                                                 zip(a.boundary_faces, b.boundary_faces)]:
+            assert np.array_equal(Fa.face_cell_conn, Fb.face_cell_conn)
+            assert np.array_equal(Fa.face_normal, Fb.face_normal)
+            assert np.array_equal(Fa.permute_vector, Fb.permute_vector)
+        assert np.array_equal(a.cell_volumes, b.cell_volumes)
