@@ -129,9 +129,16 @@ def precision_errors(U, R) -> dict:
             **{n: float(np.linalg.norm(U[:, k] - R[:, k]) / scale[k]) for k, n in enumerate(names)}}
 
 
-def timed_run(solver, steps: int, warmup: int):
+def timed_run(solver, steps: int, warmup: int, graph: int = 0):
     solver.initialize()
+    if graph:
+        solver.set_graph(graph)
     solver.step(warmup)
+    if graph:
+        # capture the graphs of both ping-pong parities before timing
+        solver.step(graph)
+        solver.step(1)
+        solver.step(graph)
     solver.synchronize()
     t0 = time.perf_counter()
     solver.step(steps)
@@ -168,10 +175,14 @@ def bench_single(args, M, opts, copy_gbs) -> dict:
     print(f"[bench] {args.workload}: {nc} cells, {nif} internal + {nbf} boundary faces, "
           f"setup {setup_s:.1f} s, device {sizes['device_bytes'] / 2**30:.2f} GiB", file=sys.stderr,
           flush=True)
-    elapsed = timed_run(solver, args.steps, args.warmup)
+    shape, tuned_ms = solver.tile_shape()
+    print(f"[bench] tile shape {shape}" + (f" (autotuned, {tuned_ms:.3f} ms/step/Mcell)" if tuned_ms else ""),
+          file=sys.stderr, flush=True)
+    elapsed = timed_run(solver, args.steps, args.warmup, args.graph)
     if solver.nonfinite():
         raise RuntimeError("non-finite state after the timed steps")
     print(f"[bench] {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    warm_total = args.warmup + (2 * args.graph + 1 if args.graph else 0)
 
     extra = {}
     if args.compare_fp64 and opts.precision != 0:
@@ -179,12 +190,12 @@ def bench_single(args, M, opts, copy_gbs) -> dict:
         U = solver.state()
         ref = M.TimeSolverExplicitRK4(md, opts.replace(precision=0))
         ref.initialize()
-        ref.step(args.warmup + args.steps)
+        ref.step(warm_total + args.steps)
         err = precision_errors(U, ref.state())
         ref.close()
         extra["precision_error"] = {
             "metric": "relative L2 vs the FP64 run after warmup+steps; momentum components "
-                      "scaled by the momentum-vector norm", "steps": args.warmup + args.steps, **err}
+                      "scaled by the momentum-vector norm", "steps": warm_total + args.steps, **err}
         print(f"[bench] precision error: {json.dumps(extra['precision_error'])}", file=sys.stderr, flush=True)
 
     so = bool(opts.second_order_space)
@@ -203,7 +214,9 @@ def bench_single(args, M, opts, copy_gbs) -> dict:
     config = {"workload": args.workload, "mesh": f"{opts.nx}x{opts.ny}x{opts.nz}", "cells": nc,
               "internal_faces": nif, "boundary_faces": nbf, "physics": physics,
               "cfl": opts.cfl, "last_dt": dt_last,
-              "precision": "fp64" if opts.precision == 0 else "fp32 flux/gradient/limiter, fp64 state"}
+              "precision": "fp64" if opts.precision == 0 else "fp32 flux/gradient/limiter, fp64 state",
+              "tile_shape": list(shape), "autotuned_ms_per_step_per_Mcell": tuned_ms or None,
+              "graph_steps": args.graph}
     return dict(value=nc * 4 * args.steps / elapsed, elapsed=elapsed, config=config, roofline=roof,
                 cpu_baseline=cpu, fp32=opts.precision != 0, extra=extra)
 
@@ -221,24 +234,28 @@ def bench_batch(args, M, cases, copy_gbs) -> dict:
     print(f"[bench] {args.workload}: {len(cases)} cases, {nc} cells, {nif} internal + {nbf} boundary "
           f"faces, mesh generation {gen_s:.1f} s, setup {setup_s:.1f} s, device "
           f"{sizes['device_bytes'] / 2**30:.2f} GiB", file=sys.stderr, flush=True)
-    elapsed = timed_run(solver, args.steps, args.warmup)
+    shape, tuned_ms = solver.tile_shape()
+    print(f"[bench] tile shape {shape}" + (f" (autotuned, {tuned_ms:.3f} ms/step/Mcell)" if tuned_ms else ""),
+          file=sys.stderr, flush=True)
+    elapsed = timed_run(solver, args.steps, args.warmup, args.graph)
     if solver.nonfinite():
         raise RuntimeError("non-finite state after the timed steps")
     print(f"[bench] {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    warm_total = args.warmup + (2 * args.graph + 1 if args.graph else 0)
     extra = {}
     fp32 = cases[0].precision != 0
     if args.compare_fp64 and fp32:
         U = [solver.case_state(k) for k in range(len(cases))]
-        ref = M.BatchSolver(list(zip(meshes, [o.replace(precision=0) for o in cases])))
+        ref = M.BatchSolver(list(zip(meshes, [o.replace(precision=0, autotune=0) for o in cases])))
         ref.initialize()
-        ref.step(args.warmup + args.steps)
+        ref.step(warm_total + args.steps)
         errs = [precision_errors(U[k], ref.case_state(k)) for k in range(len(cases))]
         ref.close()
         worst = {k: max(e[k] for e in errs) for k in errs[0]}
         extra["precision_error"] = {
             "metric": "relative L2 vs the FP64 run after warmup+steps, worst case of the batch; "
                       "momentum components scaled by the momentum-vector norm",
-            "steps": args.warmup + args.steps, **worst}
+            "steps": warm_total + args.steps, **worst}
         print(f"[bench] precision error: {json.dumps(extra['precision_error'])}", file=sys.stderr, flush=True)
     del meshes
     roof = roofline(solver, args.profile_steps, nc, nif, nbf, True, fp32,
@@ -260,6 +277,8 @@ def bench_batch(args, M, cases, copy_gbs) -> dict:
               "physics": "inviscid Roe, Green-Gauss, Van Albada, 2nd order, Sod-type states",
               "dt": cases[0].dt, "steps_per_case": cases[0].ntimesteps,
               "precision": "fp32 flux/gradient/limiter, fp64 state" if fp32 else "fp64",
+              "tile_shape": list(shape), "autotuned_ms_per_step_per_Mcell": tuned_ms or None,
+              "graph_steps": args.graph,
               "case_latency_ms": {"p50": round(per_case_ms, 2), "p99": round(per_case_ms, 2),
                                   "note": "all cases run in the same launches and finish together"}}
     return dict(value=nc * 4 * args.steps / elapsed, elapsed=elapsed, config=config, roofline=roof,
@@ -278,6 +297,12 @@ def main() -> int:
                     help="override the workload's arithmetic (0 FP64, 1 FP32 flux/gradient/limiter)")
     ap.add_argument("--compare-fp64", action="store_true",
                     help="also run the workload in FP64 and report the per-variable relative L2 error")
+    ap.add_argument("--graph", type=int, default=None,
+                    help="steps per captured HIP graph (even; 0 = plain launches); default 2 for "
+                         "batched workloads, 0 otherwise")
+    ap.add_argument("--autotune", type=int, default=None,
+                    help="1: pick the flux tile shape by timing candidates at setup (default: the "
+                         "workload's setting)")
     ap.add_argument("--copy-ceiling", action="store_true",
                     help="only measure the plain copy kernel (1 GiB, read + write) and exit")
     args = ap.parse_args()
@@ -297,16 +322,22 @@ def main() -> int:
         print(json.dumps({"copy_bytes": copy_bytes, "ms_per_copy": copy_ms, "GBps_read_plus_write": copy_gbs}))
         return 0
 
+    if args.graph is None:
+        args.graph = 2 if args.workload in BATCH_WORKLOADS else 0
     try:
         if args.workload in BATCH_WORKLOADS:
             cases = BATCH_WORKLOADS[args.workload]()
             if args.precision is not None:
                 cases = [o.replace(precision=args.precision) for o in cases]
+            if args.autotune is not None:
+                cases = [o.replace(autotune=args.autotune) for o in cases]
             r = bench_batch(args, M, cases, copy_gbs)
         else:
             opts = WORKLOADS[args.workload]()
             if args.precision is not None:
                 opts = opts.replace(precision=args.precision)
+            if args.autotune is not None:
+                opts = opts.replace(autotune=args.autotune)
             r = bench_single(args, M, opts, copy_gbs)
     except RuntimeError as e:
         print(f"[bench] {e}", file=sys.stderr)

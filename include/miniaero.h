@@ -75,6 +75,12 @@ typedef struct ma_options {
   double perturb;            /* MA_IC_PERTURBED_INFLOW: relative amplitude */
   unsigned long long seed;   /* MA_IC_PERTURBED_INFLOW: generator key */
   int precision;             /* MA_PRECISION_* */
+  /* Flux tiles: cells per workgroup as an i x j x k brick of a lattice mesh
+   * (extents >= 1, product <= 256); all 0 = 4 x 8 x 8; tile_shape[0] < 0 =
+   * 256 consecutive cells of the caller's order.  Meshes that are not a
+   * lattice always use consecutive cells.  Results do not depend on it. */
+  int tile_shape[3];
+  int autotune;              /* 1: time candidate tile shapes at create, keep the fastest */
 } ma_options;
 
 /* One Faces<Device> (Faces.h:42-71), as host arrays. */
@@ -153,6 +159,16 @@ ma_status ma_solver_set_state(ma_solver* s, const double* U);
 ma_status ma_solver_step(ma_solver* s, int nsteps);
 /* Blocks until the solver's stream is idle. */
 ma_status ma_solver_synchronize(ma_solver* s);
+/* Replays of a captured HIP graph: with steps_per_graph > 0 (an even count),
+ * ma_solver_step runs whole groups of steps_per_graph steps as one graph
+ * launch (captured on first use) and the remainder as plain launches; 0 (the
+ * default) launches every kernel separately.  Same results either way.
+ * No reference counterpart. */
+ma_status ma_solver_set_graph(ma_solver* s, int steps_per_graph);
+/* Tile shape in use (-1 -1 -1 for consecutive cells) and, after autotuning,
+ * the measured ms per step per million cells of that shape (else 0).
+ * No reference counterpart. */
+ma_status ma_solver_tile_shape(ma_solver* s, int* shape, double* tuned_ms);
 /* Copies the current solution (sol_n) to host, num_owned_cells x 5. */
 ma_status ma_solver_get_state(ma_solver* s, double* U);
 /* Residual of the last stage, num_owned_cells x 5. */

@@ -32,7 +32,8 @@ EXPORTS = (
     "ma_solver_get_gradients", "ma_solver_get_time", "ma_solver_check_finite",
     "ma_solver_stream", "ma_solver_destroy", "ma_solver_profile", "ma_solver_sizes",
     "ma_measure_copy", "ma_mesh_plan", "ma_batch_create", "ma_batch_sizes", "ma_batch_get_case_state",
-    "ma_batch_get_case_time", "ma_batch_plan",
+    "ma_batch_get_case_time", "ma_batch_plan", "ma_solver_set_graph",
+    "ma_solver_tile_shape",
 )
 
 
@@ -66,6 +67,8 @@ class ma_options(ctypes.Structure):
         ("perturb", ctypes.c_double),
         ("seed", ctypes.c_ulonglong),
         ("precision", ctypes.c_int),
+        ("tile_shape", ctypes.c_int * 3),
+        ("autotune", ctypes.c_int),
     ]
 
 
@@ -141,6 +144,8 @@ def load() -> ctypes.CDLL:
                                   ctypes.POINTER(ma_options), PP]
     L.ma_batch_plan.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.POINTER(ma_mesh_data)),
                                 ctypes.POINTER(ctypes.c_longlong)]
+    L.ma_solver_set_graph.argtypes = [P, ctypes.c_int]
+    L.ma_solver_tile_shape.argtypes = [P, _I, _D]
     L.ma_batch_sizes.argtypes = [P, _I, _I]
     L.ma_batch_get_case_state.argtypes = [P, ctypes.c_int, _D]
     L.ma_batch_get_case_time.argtypes = [P, ctypes.c_int, _D, _D]

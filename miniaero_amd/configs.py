@@ -72,7 +72,8 @@ def batched_c4(ncases: int = 512, seed: int = 4, **kw) -> list:
     on [0,1]x[0,0.25]x[0,0.25]; left/right density and pressure U[0.1, 1],
     interface position U[0.3, 0.7]; Extrapolate on every side, second order,
     Van Albada, dt 1e-4 (largest sound speed sqrt(1.4 * 1 / 0.1) = 3.74 on
-    dx >= 1/160 gives a Courant number below 0.06), 100 steps.  Draws from
+    dx >= 1/160 gives a Courant number below 0.06), 100 steps; flux tile
+    shape picked by the autotuner.  Draws from
     numpy's default_rng(seed) in the order N, rho_L, rho_R, p_L, p_R, x0."""
     rng = np.random.default_rng(seed)
     n = rng.uniform(math.log(8192.0), math.log(262144.0), ncases)
@@ -85,7 +86,7 @@ def batched_c4(ncases: int = 512, seed: int = 4, **kw) -> list:
         o = Options(problem_type=0, lx=1.0, ly=0.25, lz=0.25, nx=4 * m, ny=m, nz=m, ntimesteps=100,
                     dt=1e-4, second_order_space=1, viscous=0, limiter=1, ic_kind=1,
                     ic=(float(rho_l[i]), float(p_l[i]), float(rho_r[i]), float(p_r[i]), float(x0[i])),
-                    bc_kind=(1, 1, 1, 1, 1, 1))
+                    bc_kind=(1, 1, 1, 1, 1, 1), autotune=1)
         out.append(o.replace(**kw))
     return out
 

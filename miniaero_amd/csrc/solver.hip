@@ -673,6 +673,19 @@ __global__ __launch_bounds__(kBlock) void k_copy(long long n2, const double2* __
       return ma::fail(MA_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(err_)); \
   } while (0)
 
+// Cells per flux tile: a brick of b[0] x b[1] x b[2] lattice cells (i, j, k
+// extents, product <= kTile), or kTile consecutive cells of the caller's order
+// when all three are zero.
+struct TileShape {
+  int b[3];
+};
+constexpr TileShape kDefaultShape = {{4, 8, 8}};
+
+bool valid_shape(const TileShape& t) {
+  if (t.b[0] == 0 && t.b[1] == 0 && t.b[2] == 0) return true;
+  return t.b[0] >= 1 && t.b[1] >= 1 && t.b[2] >= 1 && t.b[0] * t.b[1] * t.b[2] <= kTile;
+}
+
 struct Timing {
   bool on = false;
   struct Rec {
@@ -711,6 +724,13 @@ struct ma_solver {
   unsigned long long* counter = nullptr;
   int cur = 0;  // U[cur] holds sol_n
   long long steps_done = 0;
+  // HIP graphs of graph_steps whole RK4 steps (an even count, so U[cur] is
+  // sol_n again after a replay), one per starting value of cur.
+  int graph_steps = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  TileShape shape = kDefaultShape;  // flux tile shape in use
+  bool all_lattice = true;
+  double tuned_ms = 0.0;  // autotuner: measured ms per step per million cells of the chosen shape
   Timing timing;
 
   template <class T>
@@ -844,20 +864,22 @@ bool lattice_coords(const ma_mesh_data* md, std::vector<int> ijk[3], int dims[3]
   return true;
 }
 
-// Device cell order: bricks of kBrick[0] x kBrick[1] x kBrick[2] lattice cells
+// Device cell order: bricks of the tile shape's lattice cells
 // (brick-lexicographic, last axis fastest, and the same inside a brick), so a
-// 256-cell tile is a compact block whose neighbours are mostly inside it.
-// Without a lattice the original order is kept.  Fills dev_of (original ->
-// device) and the cell range of every tile.
-void cell_order(const ma_mesh_data* md, std::vector<int>& dev_of, std::vector<int>& tile_cptr,
-                int dims[3]) {
+// tile is a compact block whose neighbours are mostly inside it.  Without a
+// lattice, or with the all-zero shape, the original order is kept.  Fills
+// dev_of (original -> device) and the cell range of every tile.
+void cell_order(const ma_mesh_data* md, const TileShape& shape, std::vector<int>& dev_of,
+                std::vector<int>& tile_cptr, int dims[3]) {
   const int nc = md->num_owned_cells;
-  static const int kBrick[3] = {4, 8, 8};
+  const int* kBrick = shape.b;
   std::vector<int> ijk[3];
   dev_of.resize(nc);
   tile_cptr.assign(1, 0);
-  if (!lattice_coords(md, ijk, dims)) {
-    dims[0] = dims[1] = dims[2] = 0;
+  const bool bricks = kBrick[0] > 0;
+  const bool lattice = lattice_coords(md, ijk, dims);
+  if (!lattice) dims[0] = dims[1] = dims[2] = 0;
+  if (!lattice || !bricks) {
     for (int c = 0; c < nc; ++c) dev_of[c] = c;
     for (int b = kTile; b < nc; b += kTile) tile_cptr.push_back(b);
     tile_cptr.push_back(nc);
@@ -1030,7 +1052,7 @@ void merge_layouts(std::vector<Layout>& parts, Layout& M, std::vector<int>& tile
   }
 }
 
-ma_status plan_layout(const ma_mesh_data* md, Layout& L) {
+ma_status plan_layout(const ma_mesh_data* md, const TileShape& shape, Layout& L) {
   const int nc = md->num_owned_cells;
   L.nc = nc;
   const ma_faces& I = md->internal_faces;
@@ -1055,7 +1077,7 @@ ma_status plan_layout(const ma_mesh_data* md, Layout& L) {
   // Cell order on the device, and its inverse.
   std::vector<int>& dev_of = L.dev_of;
   std::vector<int>& tile_cptr = L.tile_cptr;
-  cell_order(md, dev_of, tile_cptr, L.dims);
+  cell_order(md, shape, dev_of, tile_cptr, L.dims);
   std::vector<int>& org_of = L.org_of;
   org_of.assign(nc, 0);
   for (int c = 0; c < nc; ++c) org_of[dev_of[c]] = c;
@@ -1250,10 +1272,12 @@ ma_status check_mesh(const ma_mesh_data* md, const char* who) {
 // checked, with the case of every tile, boundary face and device cell.
 struct RunPlan {
   Layout L;
+  bool all_lattice = true;  // every case's mesh is an i x j x k lattice
   std::vector<int> tile_case, case_tiles, bc_case, cell_case, case_cell_off;
 };
 
-ma_status plan_run(int ncases, const ma_mesh_data* const* mds, RunPlan& P) {
+ma_status plan_run(int ncases, const ma_mesh_data* const* mds, const TileShape& shape, RunPlan& P) {
+  if (!valid_shape(shape)) return ma::fail(MA_ERR_INVALID, "tile shape: extents >= 1 with product <= 256, or all 0");
   P.case_cell_off.assign(ncases + 1, 0);
   long long total_cells = 0, total_faces = 0;
   for (int k = 0; k < ncases; ++k) {
@@ -1269,7 +1293,8 @@ ma_status plan_run(int ncases, const ma_mesh_data* const* mds, RunPlan& P) {
   ma_status st = MA_OK;
   Layout& L = P.L;
   if (ncases == 1) {
-    if ((st = plan_layout(mds[0], L))) return st;
+    if ((st = plan_layout(mds[0], shape, L))) return st;
+    P.all_lattice = L.dims[0] > 0;
     P.tile_case.assign(L.ntiles, 0);
     P.case_tiles = {0, L.ntiles};
     P.bc_case.assign(std::max(1, L.nbf), 0);
@@ -1283,7 +1308,7 @@ ma_status plan_run(int ncases, const ma_mesh_data* const* mds, RunPlan& P) {
     auto worker = [&] {
       for (int k = next++; k < ncases; k = next++) {
         try {
-          status[k] = plan_layout(mds[k], parts[k]);
+          status[k] = plan_layout(mds[k], shape, parts[k]);
           if (status[k]) message[k] = ma::last_error();
         } catch (const std::bad_alloc&) {
           status[k] = MA_ERR_NOMEM;
@@ -1296,8 +1321,10 @@ ma_status plan_run(int ncases, const ma_mesh_data* const* mds, RunPlan& P) {
     for (int i = 1; i < nthreads; ++i) pool.emplace_back(worker);
     worker();
     for (auto& th : pool) th.join();
-    for (int k = 0; k < ncases; ++k)
+    for (int k = 0; k < ncases; ++k) {
       if (status[k]) return ma::fail(status[k], "case " + std::to_string(k) + ": " + message[k]);
+      P.all_lattice = P.all_lattice && parts[k].dims[0] > 0;
+    }
     merge_layouts(parts, L, P.tile_case, P.case_tiles, P.bc_case, P.cell_case);
   }
   return validate_layout(L);
@@ -1316,8 +1343,9 @@ void plan_stats(const Layout& L, long long* stats) {
 // Device arrays of one run: the layout, the per-case parameters and the work buffers.
 ma_status build(ma_solver* s, int ncases, const ma_mesh_data* const* mds) {
   RunPlan P;
-  ma_status st = plan_run(ncases, mds, P);
+  ma_status st = plan_run(ncases, mds, s->shape, P);
   if (st) return st;
+  s->all_lattice = P.all_lattice;
   Layout& L = P.L;
   std::vector<int>& tile_case = P.tile_case;
   std::vector<int>& case_tiles = P.case_tiles;
@@ -1496,6 +1524,52 @@ void dispatch(ma_solver* s, int n) {
     dispatch_t<double>(s, n);
 }
 
+void drop_graphs(ma_solver* s) {
+  for (auto& g : s->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+// Captures graph_steps steps starting from the current cur (no launch runs).
+ma_status capture_graph(ma_solver* s) {
+  const int cur = s->cur;
+  const long long done = s->steps_done;
+  hipGraph_t g = nullptr;
+  MA_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  dispatch(s, s->graph_steps);
+  const hipError_t e = hipStreamEndCapture(s->stream, &g);
+  s->cur = cur;  // nothing ran
+  s->steps_done = done;
+  if (e != hipSuccess || !g)
+    return ma::fail(MA_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  hipGraphExec_t x = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) return ma::fail(MA_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  s->graph[cur] = x;
+  return MA_OK;
+}
+
+// nsteps steps: whole graphs while at least graph_steps remain, the rest as
+// plain launches.
+ma_status advance(ma_solver* s, int nsteps) {
+  if (s->graph_steps > 0) {
+    while (nsteps >= s->graph_steps) {
+      if (!s->graph[s->cur]) {
+        ma_status st = capture_graph(s);
+        if (st) return st;
+      }
+      MA_HIP(hipGraphLaunch(s->graph[s->cur], s->stream));
+      s->steps_done += s->graph_steps;
+      nsteps -= s->graph_steps;
+    }
+  }
+  dispatch(s, nsteps);
+  return MA_OK;
+}
+
 template <class S>
 ma_status download_aos(ma_solver* s, const S* dsrc, int ncomp, double* host) {
   const int nc = s->m.nc;
@@ -1518,7 +1592,7 @@ extern "C" ma_status ma_batch_plan(int ncases, const ma_mesh_data* const* meshes
   RunPlan P;
   ma_status st = MA_OK;
   try {
-    st = plan_run(ncases, meshes, P);
+    st = plan_run(ncases, meshes, kDefaultShape, P);
   } catch (const std::bad_alloc&) {
     st = ma::fail(MA_ERR_NOMEM, "plan: out of host memory");
   }
@@ -1534,8 +1608,73 @@ extern "C" ma_status ma_mesh_plan(const ma_mesh_data* md, long long* stats) {
 
 namespace {
 
-// Shared body of ma_solver_create and ma_batch_create.
-ma_status create_run(int ncases, const ma_mesh_data* const* mds, const ma_options* opts, ma_solver** out) {
+// Tile shape requested by the options: zeros = the default brick, a negative
+// first extent = consecutive cells.
+TileShape shape_of(const ma_options& o) {
+  if (o.tile_shape[0] < 0) return TileShape{{0, 0, 0}};
+  if (o.tile_shape[0] == 0 && o.tile_shape[1] == 0 && o.tile_shape[2] == 0) return kDefaultShape;
+  return TileShape{{o.tile_shape[0], o.tile_shape[1], o.tile_shape[2]}};
+}
+
+ma_status create_run(int ncases, const ma_mesh_data* const* mds, const ma_options* opts, ma_solver** out,
+                     const TileShape* force = nullptr);
+
+// Autotuner: builds the leading cases (at least 2^21 cells, or all of them)
+// with every candidate tile shape, times a few steps of each from the
+// initial state with HIP events, and returns the fastest shape and its time
+// in ms per step per million cells.  Results do not depend on the shape
+// (every cell sums its faces in the same order whatever the tiles), only
+// the speed does.
+ma_status autotune(int ncases, const ma_mesh_data* const* mds, const ma_options* opts, TileShape* best,
+                   double* best_ms) {
+  static const TileShape cand[] = {{{4, 8, 8}}, {{8, 8, 4}}, {{8, 4, 8}}, {{4, 4, 16}}, {{16, 4, 4}},
+                                   {{2, 8, 16}}, {{8, 2, 16}}, {{4, 16, 4}}, {{2, 16, 8}}, {{4, 4, 8}},
+                                   {{0, 0, 0}}};
+  constexpr int kWarm = 2, kTimed = 4;
+  int n = 0;
+  long long cells = 0;
+  while (n < ncases && cells < (1ll << 21)) cells += mds[n++]->num_owned_cells;
+  std::vector<ma_options> so(opts, opts + n);
+  for (auto& o : so) o.autotune = 0;
+  *best_ms = 0.0;
+  *best = kDefaultShape;
+  double best_v = 1e300;
+  for (const TileShape& c : cand) {
+    ma_solver* t = nullptr;
+    ma_status st = create_run(n, mds, so.data(), &t, &c);
+    if (st == MA_ERR_UNSUPPORTED) continue;  // e.g. a tile with too many faces for LDS
+    if (st) return st;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.f;
+    bool ok = ma_solver_initialize(t) == MA_OK && hipEventCreate(&e0) == hipSuccess &&
+              hipEventCreate(&e1) == hipSuccess;
+    if (ok) {
+      dispatch(t, kWarm);
+      ok = hipEventRecord(e0, t->stream) == hipSuccess;
+      dispatch(t, kTimed);
+      ok = ok && hipEventRecord(e1, t->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+           hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+    }
+    const bool lattice = t->all_lattice;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    ma_solver_destroy(t);
+    if (!ok) return ma::fail(MA_ERR_DEVICE, "autotune: timing run failed");
+    const double v = ms / kTimed / (cells * 1e-6);
+    if (v < best_v) {
+      best_v = v;
+      *best = c;
+      *best_ms = v;
+    }
+    if (!lattice) break;  // without a lattice every shape is the same layout
+  }
+  return MA_OK;
+}
+
+// Shared body of ma_solver_create and ma_batch_create; `force` fixes the
+// tile shape (no autotuning).
+ma_status create_run(int ncases, const ma_mesh_data* const* mds, const ma_options* opts, ma_solver** out,
+                     const TileShape* force) {
   const ma_options& o0 = opts[0];
   for (int k = 0; k < ncases; ++k) {
     ma_status st = check_mesh(mds[k], "create");
@@ -1547,13 +1686,27 @@ ma_status create_run(int ncases, const ma_mesh_data* const* mds, const ma_option
       return ma::fail(MA_ERR_INVALID,
                       "batched cases must share second_order_space, viscous, limiter, precision and prandtl");
   }
+  TileShape shape = force ? *force : shape_of(o0);
+  if (!valid_shape(shape)) return ma::fail(MA_ERR_INVALID, "tile shape: extents >= 1 with product <= 256");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return ma::fail(MA_ERR_DEVICE, "no HIP device available");
+  double tuned_ms = 0.0;
+  if (!force && o0.autotune) {
+    ma_status st = MA_OK;
+    try {
+      st = autotune(ncases, mds, opts, &shape, &tuned_ms);
+    } catch (const std::bad_alloc&) {
+      st = ma::fail(MA_ERR_NOMEM, "autotune: out of host memory");
+    }
+    if (st) return st;
+  }
   ma_solver* s = new (std::nothrow) ma_solver();
   if (!s) return ma::fail(MA_ERR_NOMEM, "create: out of host memory");
   ma_status st = MA_OK;
   try {
+    s->shape = shape;
+    s->tuned_ms = tuned_ms;
     s->opt = o0;
     s->ncases = ncases;
     s->case_opt.assign(opts, opts + ncases);
@@ -1619,8 +1772,26 @@ extern "C" ma_status ma_solver_set_state(ma_solver* s, const double* U) {
 extern "C" ma_status ma_solver_step(ma_solver* s, int nsteps) {
   if (!s) return ma::fail(MA_ERR_INVALID, "null solver");
   if (nsteps < 0) return ma::fail(MA_ERR_INVALID, "negative step count");
-  dispatch(s, nsteps);
+  ma_status st = advance(s, nsteps);
+  if (st) return st;
   MA_HIP(hipGetLastError());
+  return MA_OK;
+}
+
+extern "C" ma_status ma_solver_tile_shape(ma_solver* s, int* shape, double* tuned_ms) {
+  if (!s || !shape) return ma::fail(MA_ERR_INVALID, "ma_solver_tile_shape: null argument");
+  const bool consecutive = s->shape.b[0] == 0;
+  for (int k = 0; k < 3; ++k) shape[k] = consecutive ? -1 : s->shape.b[k];
+  if (tuned_ms) *tuned_ms = s->tuned_ms;
+  return MA_OK;
+}
+
+extern "C" ma_status ma_solver_set_graph(ma_solver* s, int steps_per_graph) {
+  if (!s || steps_per_graph < 0 || (steps_per_graph & 1))
+    return ma::fail(MA_ERR_INVALID, "ma_solver_set_graph: steps_per_graph must be an even count >= 0");
+  MA_HIP(hipStreamSynchronize(s->stream));
+  drop_graphs(s);
+  s->graph_steps = steps_per_graph;
   return MA_OK;
 }
 
@@ -1812,6 +1983,7 @@ extern "C" ma_status ma_measure_copy(long long bytes, int reps, double* ms_per_c
 extern "C" void ma_solver_destroy(ma_solver* s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  drop_graphs(s);
   for (void* p : s->allocs) (void)hipFree(p);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;

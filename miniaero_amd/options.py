@@ -39,6 +39,8 @@ class Options:
     perturb: float = 0.0
     seed: int = 0
     precision: int = 0        # 0 FP64, 1 FP32 flux/gradient/limiter arithmetic
+    tile_shape: tuple = (0, 0, 0)   # flux tile brick (i, j, k); zeros = 4x8x8, (-1, ..) = consecutive
+    autotune: int = 0         # 1: pick the fastest tile shape at solver creation
 
     def read_options_file(self, path: str = "miniaero.inp") -> "Options":
         """Options::read_options_file (Options.h:73-101): nine whitespace
@@ -55,7 +57,7 @@ class Options:
         o = _lib.ma_options()
         for f in dataclasses.fields(self):
             v = getattr(self, f.name)
-            if f.name in ("bc_kind", "ic", "inflow"):
+            if f.name in ("bc_kind", "ic", "inflow", "tile_shape"):
                 arr = getattr(o, f.name)
                 for i, x in enumerate(v):
                     arr[i] = x

@@ -66,6 +66,17 @@ class TimeSolverExplicitRK4:
     def step(self, n: int = 1) -> None:
         _lib.check(self._L.ma_solver_step(self._h, int(n)))
 
+    def tile_shape(self):
+        """(tile brick (i, j, k) in use, autotuned ms per step per million cells or 0)."""
+        sh = (ctypes.c_int * 3)()
+        ms = ctypes.c_double()
+        _lib.check(self._L.ma_solver_tile_shape(self._h, sh, ctypes.byref(ms)))
+        return tuple(sh), ms.value
+
+    def set_graph(self, steps_per_graph: int) -> None:
+        """Run groups of steps_per_graph steps (even) as one captured HIP graph; 0 = off."""
+        _lib.check(self._L.ma_solver_set_graph(self._h, int(steps_per_graph)))
+
     def synchronize(self) -> None:
         _lib.check(self._L.ma_solver_synchronize(self._h))
 

@@ -117,3 +117,76 @@ def test_many_small_cases():
                         ic_kind=1, ic=(1.0, 1.0, 0.125, 0.1, float(rng.uniform(0.2, 0.8))),
                         bc_kind=(1, 1, 1, 1, 1, 1)))
     run_batch_vs_alone(kws, 6)
+
+
+@pytest.mark.parametrize("graph_steps", [2, 4])
+def test_graph_replays_match_plain_launches(graph_steps):
+    """Steps run as captured HIP graphs (several replays, from both ping-pong
+    parities, with eager work and host reads between them) equal plain
+    launches bit for bit, for a batch with fixed-dt and CFL cases."""
+    kws = [VISCOUS_CASES[1], VISCOUS_CASES[2], VISCOUS_CASES[4]]
+    cases = [make_case(kw) for kw in kws]
+    a = M.BatchSolver(cases)
+    b = M.BatchSolver(cases)
+    try:
+        a.initialize()
+        b.initialize()
+        b.set_graph(graph_steps)
+        for n in (graph_steps + 1, 3, 2 * graph_steps, 1, graph_steps):
+            a.step(n)
+            b.step(n)
+            Ua, Ub = a.state(), b.state()
+            print(f"[gpu-batch] graph {graph_steps}: +{n} steps, bitwise={np.array_equal(Ua, Ub)}",
+                  flush=True)
+            assert np.array_equal(Ua, Ub)
+            assert b.nonfinite() == 0
+            for k in range(len(cases)):
+                assert a.case_time(k) == b.case_time(k)
+        # profiling (plain launches with events) still works after graphs
+        b.profile(2)
+        a.step(2)
+        assert np.array_equal(a.state(), b.state())
+    finally:
+        a.close()
+        b.close()
+
+
+@pytest.mark.parametrize("shape", [(-1, 0, 0), (1, 1, 1), (2, 8, 16), (8, 8, 4), (16, 4, 4), (3, 5, 7)])
+def test_tile_shape_does_not_change_results(shape):
+    """Every cell sums its faces in the same order whatever the flux tiles
+    are, so any tile shape gives the default layout's bits."""
+    kw = plate(nx=20, ny=18, nz=10, y_stretch=1.05, bc_kind=(3, 2, 2, 2, 1, 4), dt=0.0, cfl=0.5)
+    md, opts = make_case(kw)
+    out = []
+    for o in (opts, opts.replace(tile_shape=shape)):
+        s = M.TimeSolverExplicitRK4(md, o)
+        try:
+            s.initialize()
+            s.step(6)
+            out.append(s.state())
+            got_shape, _ = s.tile_shape()
+        finally:
+            s.close()
+    print(f"[gpu-batch] tile shape {shape} -> {got_shape}, bitwise={np.array_equal(out[0], out[1])}",
+          flush=True)
+    assert got_shape == ((-1, -1, -1) if shape[0] < 0 else shape)
+    assert np.array_equal(out[0], out[1])
+
+
+def test_autotuned_batch_matches_default():
+    kws = VANALBADA_CASES[:3]
+    cases = [make_case(kw) for kw in kws]
+    tuned = [(md, o.replace(autotune=1)) for md, o in cases]
+    out = []
+    for cs in (cases, tuned):
+        b = M.BatchSolver(cs)
+        try:
+            b.initialize()
+            b.step(8)
+            out.append(b.state())
+            shape, ms = b.tile_shape()
+        finally:
+            b.close()
+    print(f"[gpu-batch] autotuned shape {shape}, {ms:.4f} ms/step/Mcell", flush=True)
+    assert ms > 0.0
+    assert np.array_equal(out[0], out[1])

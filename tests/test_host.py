@@ -241,3 +241,11 @@ def test_mesh_generation_on_threads_is_bit_exact():
             assert np.array_equal(Fa.face_normal, Fb.face_normal)
             assert np.array_equal(Fa.permute_vector, Fb.permute_vector)
         assert np.array_equal(a.cell_volumes, b.cell_volumes)
+
+
+def test_invalid_tile_shape_is_rejected_before_device_use():
+    md = M.Parallel3DMesh(2, 2, 2, 1, 1, 1, 0).fillMeshData()
+    for shape in ((16, 16, 2), (0, 4, 4), (4, -2, 4)):
+        with pytest.raises(M.MiniAeroError) as e:
+            M.TimeSolverExplicitRK4(md, M.Options(dt=1e-6, tile_shape=shape))
+        assert e.value.status == _lib.MA_ERR_INVALID
