@@ -285,6 +285,100 @@ def bench_batch(args, M, cases, copy_gbs) -> dict:
                 cpu_baseline=cpu, fp32=fp32, extra=extra)
 
 
+def bench_served(args, M, stream, copy_gbs) -> dict:
+    """Replays the arrival stream in real time against a Server: each request
+    is submitted at its arrival time, finished cases are collected as they
+    come.  value = sum over cases of cells * 4 * steps / (last completion -
+    first submission); latency = completion - submission."""
+    import numpy as np
+    srv = M.Server(hbm_budget_bytes=args.hbm_budget, max_active=args.max_active)
+    # one small case first, so that module loading is not in the first request's latency
+    warm = stream[0][1].replace(nx=8, ny=4, nz=2, ntimesteps=2)
+    wid = srv.submit(warm)
+    while True:
+        got = srv.poll(timeout=5.0)
+        if got:
+            if got[0]["status"] != 0:
+                raise RuntimeError(f"warm-up case failed: {got[0]}")
+            srv.release(wid)
+            break
+    st0 = srv.stats()
+    print(f"[bench] served: {len(stream)} requests, region {st0['region_size'] / 1e9:.1f} GB, "
+          f"max_active {args.max_active or 16}", file=sys.stderr, flush=True)
+    done, opts_of = {}, {}
+    t0 = srv.now()
+    last_note = time.time()
+    peak_used = 0
+
+    def collect(timeout):
+        nonlocal peak_used
+        for r in srv.poll(timeout=timeout):
+            done[r["id"]] = r
+            if args.compare_fp64 and r["status"] == 0 and len([1 for x in done.values() if "U" in x]) < 6:
+                r["U"] = srv.state(r["id"], r["cells"])
+            srv.release(r["id"])
+        peak_used = max(peak_used, srv.stats()["region_used"])
+
+    for arrival, o in stream:
+        while True:
+            wait = t0 + arrival - srv.now()
+            if wait <= 0:
+                break
+            collect(min(wait, 0.005))
+        opts_of[srv.submit(o)] = o
+        if time.time() - last_note > 10:
+            print(f"[bench] {len(opts_of)} submitted, {len(done)} finished", file=sys.stderr, flush=True)
+            last_note = time.time()
+    while len(done) < len(opts_of):
+        collect(1.0)
+        if time.time() - last_note > 10:
+            print(f"[bench] {len(done)} of {len(opts_of)} finished", file=sys.stderr, flush=True)
+            last_note = time.time()
+    srv.close()
+    bad = [r for r in done.values() if r["status"] != 0]
+    if bad:
+        raise RuntimeError(f"{len(bad)} served cases failed, e.g. {bad[0]}")
+    rs = list(done.values())
+    lat = np.array([r["t_finish"] - r["t_submit"] for r in rs])
+    wait = np.array([r["t_admit"] - r["t_submit"] for r in rs])
+    work = sum(r["cells"] * 4 * r["steps"] for r in rs)
+    first = min(r["t_submit"] for r in rs)
+    span = max(r["t_finish"] for r in rs) - first
+    extra = {}
+    if args.compare_fp64:
+        errs = []
+        for r in rs:
+            if "U" not in r:
+                continue
+            o = opts_of[r["id"]].replace(precision=0)
+            md = M.Parallel3DMesh(o.nx, o.ny, o.nz, o.lx, o.ly, o.lz, o.problem_type, o.angle,
+                                  options=o).fillMeshData()
+            ref = M.TimeSolverExplicitRK4(md, o)
+            ref.initialize()
+            ref.step(o.ntimesteps)
+            errs.append(precision_errors(r["U"], ref.state()))
+            ref.close()
+        if errs:
+            extra["precision_error"] = {
+                "metric": f"relative L2 vs FP64 runs of the first {len(errs)} finished cases, worst case; "
+                          "momentum components scaled by the momentum-vector norm",
+                **{k: max(e[k] for e in errs) for k in errs[0]}}
+    cells = np.array([r["cells"] for r in rs])
+    config = {"workload": args.workload, "requests": len(rs),
+              "arrivals": "Poisson, 200 per second (seeded)", "cells_per_case":
+                  {"min": int(cells.min()), "mean": float(cells.mean()), "max": int(cells.max())},
+              "physics": "Roe, Green-Gauss, Venkat, 2nd order; half viscous flat plate, half slip wall",
+              "precision": "fp32 flux/gradient/limiter, fp64 state",
+              "max_active": args.max_active or 16, "region_bytes": st0["region_size"],
+              "peak_region_bytes_used": int(peak_used),
+              "case_latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                                 "max": float(lat.max())},
+              "admission_wait_s": {"p50": float(np.percentile(wait, 50)), "p99": float(np.percentile(wait, 99))},
+              "span_s": span, "offered_cell_stages_per_s": work / max(stream[-1][0] - stream[0][0], 1e-9)}
+    return dict(value=work / span, elapsed=span, config=config, roofline=None, cpu_baseline=None, fp32=True,
+                extra=extra, steps=None)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -303,6 +397,9 @@ def main() -> int:
     ap.add_argument("--autotune", type=int, default=None,
                     help="1: pick the flux tile shape by timing candidates at setup (default: the "
                          "workload's setting)")
+    ap.add_argument("--cases", type=int, default=200, help="served workload: requests in the stream")
+    ap.add_argument("--max-active", type=int, default=0, help="served workload: cases on the GPU at once")
+    ap.add_argument("--hbm-budget", type=float, default=200e9, help="served workload: device bytes for cases")
     ap.add_argument("--copy-ceiling", action="store_true",
                     help="only measure the plain copy kernel (1 GiB, read + write) and exit")
     args = ap.parse_args()
@@ -313,7 +410,7 @@ def main() -> int:
 
     import miniaero_amd as M
     from miniaero_amd import _lib
-    from miniaero_amd.configs import BATCH_WORKLOADS, WORKLOADS
+    from miniaero_amd.configs import BATCH_WORKLOADS, SERVED_WORKLOADS, WORKLOADS
 
     copy_bytes = 1 << 30
     copy_ms = _lib.measure_copy(copy_bytes, 10)
@@ -325,7 +422,9 @@ def main() -> int:
     if args.graph is None:
         args.graph = 2 if args.workload in BATCH_WORKLOADS else 0
     try:
-        if args.workload in BATCH_WORKLOADS:
+        if args.workload in SERVED_WORKLOADS:
+            r = bench_served(args, M, SERVED_WORKLOADS[args.workload](args.cases), copy_gbs)
+        elif args.workload in BATCH_WORKLOADS:
             cases = BATCH_WORKLOADS[args.workload]()
             if args.precision is not None:
                 cases = [o.replace(precision=args.precision) for o in cases]
@@ -347,9 +446,9 @@ def main() -> int:
         "value": r["value"],
         "unit": "cell-stage updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": args.steps if r.get("steps", 1) is not None else None,
         "warmup": args.warmup,
-        "ms_per_step": r["elapsed"] / args.steps * 1e3,
+        "ms_per_step": r["elapsed"] / args.steps * 1e3 if r.get("steps", 1) is not None else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

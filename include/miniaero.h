@@ -206,6 +206,55 @@ ma_status ma_batch_get_case_state(ma_solver* s, int k, double* U);
 /* Time and last dt of case k (dt differs between CFL-driven cases). */
 ma_status ma_batch_get_case_time(ma_solver* s, int k, double* time, double* last_dt);
 
+/* ---- served form: a request loop over independent cases ---- */
+
+/* A server keeps admitted cases' state on the device between steps and runs
+ * them concurrently, each on its own stream, while new requests arrive
+ * (continuous admission).  Requests are planned on host threads (mesh
+ * generated from the options as Parallel3DMesh does, layout, exact device
+ * size) and admitted in submission order when a slot is free and their
+ * device bytes fit in the server's region of HBM, which is allocated once at
+ * creation.  Each case runs opts->ntimesteps RK4 steps from its initial
+ * state; its result equals a ma_solver_create run of the same case alone,
+ * bit for bit.  No reference counterpart (the reference runs one case per
+ * process, Main.C:96-141). */
+typedef struct ma_server ma_server;
+
+typedef struct {
+  long long hbm_budget_bytes; /* device region for admitted cases; 0 = 200e9, capped by free HBM */
+  int max_active;             /* cases on the device at once; 0 = 16 */
+  int setup_threads;          /* host threads planning requests; 0 = 4 */
+  int steps_per_launch;       /* RK4 steps per captured graph launch (even); 0 = 10 */
+} ma_server_config;
+
+typedef struct {
+  long long id;
+  int status;              /* ma_status of the case; MA_OK when it ran to the end */
+  int steps;               /* steps run */
+  long long cells;
+  long long device_bytes;  /* bytes of the case in the server's region */
+  double t_submit, t_admit, t_finish;  /* seconds on the server's clock (ma_server_now) */
+} ma_case_result;
+
+ma_status ma_server_create(const ma_server_config* cfg, ma_server** out);
+/* Queues a case; *id identifies it in results.  Never blocks on the device. */
+ma_status ma_server_submit(ma_server* srv, const ma_options* opts, long long* id);
+/* Moves up to max finished cases (in completion order) into out, waiting up
+ * to timeout_s seconds for the first; returns the count (-1 on error). */
+int ma_server_poll(ma_server* srv, ma_case_result* out, int max, double timeout_s);
+/* Final state of a finished case (cells x 5 conserved values, the generated
+ * mesh's cell order); kept on the host until ma_server_release. */
+ma_status ma_server_get_state(ma_server* srv, long long id, double* U);
+ma_status ma_server_release(ma_server* srv, long long id);
+/* stats[0] submitted, [1] waiting (planning or for admission), [2] active,
+ * [3] finished, [4] bytes of the region in use, [5] region size. */
+ma_status ma_server_stats(ma_server* srv, long long* stats);
+/* Seconds since the server was created. */
+double ma_server_now(ma_server* srv);
+/* Cancels cases not yet admitted, waits for the active ones to finish their
+ * launched steps, frees everything. */
+void ma_server_destroy(ma_server* srv);
+
 /* ---- measurement ---- */
 
 /* Per-kernel timing: runs nsteps with a HIP event pair around every launch on

@@ -13,10 +13,11 @@ from ._lib import MiniAeroError, load
 from .mesh import Faces, MeshData, Parallel3DMesh, plan_batch
 from .options import Options
 from .results import numeric_file_diff, numeric_text_diff, read_results, write_results
+from .server import Server
 from .solver import BatchSolver, TimeSolverExplicitRK4
 
 __all__ = [
     "MiniAeroError", "load", "Faces", "MeshData", "Parallel3DMesh", "Options",
-    "TimeSolverExplicitRK4", "BatchSolver", "numeric_text_diff", "numeric_file_diff", "read_results",
+    "TimeSolverExplicitRK4", "BatchSolver", "Server", "numeric_text_diff", "numeric_file_diff", "read_results",
     "write_results",
 ]

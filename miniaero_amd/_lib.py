@@ -33,8 +33,20 @@ EXPORTS = (
     "ma_solver_stream", "ma_solver_destroy", "ma_solver_profile", "ma_solver_sizes",
     "ma_measure_copy", "ma_mesh_plan", "ma_batch_create", "ma_batch_sizes", "ma_batch_get_case_state",
     "ma_batch_get_case_time", "ma_batch_plan", "ma_solver_set_graph",
-    "ma_solver_tile_shape",
+    "ma_solver_tile_shape", "ma_server_create", "ma_server_submit", "ma_server_poll",
+    "ma_server_get_state", "ma_server_release", "ma_server_stats", "ma_server_now", "ma_server_destroy",
 )
+
+
+class ma_server_config(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_longlong), ("max_active", ctypes.c_int),
+                ("setup_threads", ctypes.c_int), ("steps_per_launch", ctypes.c_int)]
+
+
+class ma_case_result(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_longlong), ("status", ctypes.c_int), ("steps", ctypes.c_int),
+                ("cells", ctypes.c_longlong), ("device_bytes", ctypes.c_longlong),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_finish", ctypes.c_double)]
 
 
 class MiniAeroError(RuntimeError):
@@ -146,6 +158,17 @@ def load() -> ctypes.CDLL:
                                 ctypes.POINTER(ctypes.c_longlong)]
     L.ma_solver_set_graph.argtypes = [P, ctypes.c_int]
     L.ma_solver_tile_shape.argtypes = [P, _I, _D]
+    L.ma_server_create.argtypes = [ctypes.POINTER(ma_server_config), PP]
+    L.ma_server_submit.argtypes = [P, ctypes.POINTER(ma_options), ctypes.POINTER(ctypes.c_longlong)]
+    L.ma_server_poll.restype = ctypes.c_int
+    L.ma_server_poll.argtypes = [P, ctypes.POINTER(ma_case_result), ctypes.c_int, ctypes.c_double]
+    L.ma_server_get_state.argtypes = [P, ctypes.c_longlong, _D]
+    L.ma_server_release.argtypes = [P, ctypes.c_longlong]
+    L.ma_server_stats.argtypes = [P, ctypes.POINTER(ctypes.c_longlong)]
+    L.ma_server_now.restype = ctypes.c_double
+    L.ma_server_now.argtypes = [P]
+    L.ma_server_destroy.argtypes = [P]
+    L.ma_server_destroy.restype = None
     L.ma_batch_sizes.argtypes = [P, _I, _I]
     L.ma_batch_get_case_state.argtypes = [P, ctypes.c_int, _D]
     L.ma_batch_get_case_time.argtypes = [P, ctypes.c_int, _D, _D]

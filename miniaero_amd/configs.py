@@ -91,5 +91,41 @@ def batched_c4(ncases: int = 512, seed: int = 4, **kw) -> list:
     return out
 
 
+def served_case(viscous: bool, mach: float, angle_deg: float, cells: int, steps: int) -> Options:
+    """One request of the served stream: the C2 flat-plate box (4:2:1 cells,
+    m = round((cells/8)^(1/3)), nx = 4m, ny = 2m, nz = m) with a freestream
+    at the given Mach number and inflow angle, T 300 K, Re 1e5 per metre,
+    CFL 0.5, second order with Venkat, FP32 flux/gradient/limiter arithmetic
+    and FP64 state.  Viscous: the C2 boundaries and y-stretching.  Inviscid:
+    the same box with a slip (Tangent) wall at -y and uniform spacing."""
+    m = max(1, int(round((cells / 8.0) ** (1.0 / 3.0))))
+    o = flat_plate_c2(nx=4 * m, ny=2 * m, nz=m, ntimesteps=int(steps),
+                      inflow=freestream_state(mach, 300.0, 1.0e5, angle_deg), precision=1)
+    if not viscous:
+        o = o.replace(viscous=0, y_stretch=0.0, bc_kind=(2, 2, 2, 2, 1, 4))
+    return o
+
+
+def served_c5(ncases: int = 2000, seed: int = 5) -> list:
+    """The served stream: (arrival time in s, Options) for ncases requests.
+    Poisson arrivals at 200 per second (exponential gaps, mean 5 ms), viscous
+    or inviscid with probability 1/2, cells log-uniform over [4096, 2097152],
+    Mach U[0.2, 0.8], inflow angle U[-5, 5] degrees, steps uniform over
+    50..500.  Draws from numpy's default_rng(seed) in the order gaps, kind,
+    cells, Mach, angle, steps."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / 200.0, ncases)
+    viscous = rng.random(ncases) < 0.5
+    cells = np.exp(rng.uniform(math.log(4096.0), math.log(2097152.0), ncases))
+    mach = rng.uniform(0.2, 0.8, ncases)
+    angle = rng.uniform(-5.0, 5.0, ncases)
+    steps = rng.integers(50, 501, ncases)
+    arrival = np.cumsum(gaps)
+    return [(float(arrival[i]), served_case(bool(viscous[i]), float(mach[i]), float(angle[i]),
+                                            float(cells[i]), int(steps[i])))
+            for i in range(ncases)]
+
+
 WORKLOADS = {"sod_c1": sod_c1, "flat_plate_c2": flat_plate_c2, "flat_plate_c3": flat_plate_c3}
 BATCH_WORKLOADS = {"batched_c4": batched_c4}
+SERVED_WORKLOADS = {"served_c5": served_c5}

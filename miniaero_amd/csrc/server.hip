@@ -1,0 +1,497 @@
+/*
+ * server.hip -- served form: independent cases admitted while others run.
+ *
+ * No reference counterpart: the reference runs one case per process
+ * (Main.C:96-141).  The request loop here:
+ *
+ *   submit -> queue -> setup thread: generate the mesh from the options
+ *   (ma_mesh_generate), plan the device layout and compute the exact device
+ *   bytes (ma::plan_case) -> admission in submission order, when a slot is
+ *   free and the bytes fit in the server's region -> build inside the region
+ *   on the case's own stream, initial state, graph capture -> scheduler
+ *   thread: keeps at most two launches of steps_per_launch steps in flight
+ *   per case, so every admitted case has queued work and cases overlap on
+ *   the device -> when a case has run its steps, its state is copied to the
+ *   host, its region block is returned and the next request can be admitted.
+ *
+ * The region is one hipMalloc made at creation, so admitting and retiring
+ * cases never allocates or frees device memory (hipFree can wait for the
+ * whole device, which would stall the other cases).  Each case owns its
+ * buffers and stream and runs the same kernels in the same order as a
+ * ma_solver_create run of it alone, so its result is bit for bit the same.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/miniaero.h"
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// First-fit allocator of [0, size) in 2 MiB granules, coalescing on release.
+class Region {
+ public:
+  static size_t granule(size_t n) {
+    const size_t g = size_t(2) << 20;
+    return (n + g - 1) / g * g;
+  }
+  void reset(size_t size) {
+    free_.clear();
+    if (size) free_[0] = size;
+    size_ = size;
+    used_ = 0;
+  }
+  bool reserve(size_t n, size_t* off) {
+    n = granule(n);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < n) continue;
+      *off = it->first;
+      const size_t rest = it->second - n, at = it->first + n;
+      free_.erase(it);
+      if (rest) free_[at] = rest;
+      used_ += n;
+      return true;
+    }
+    return false;
+  }
+  void release(size_t off, size_t n) {
+    n = granule(n);
+    used_ -= n;
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  size_t used() const { return used_; }
+  size_t size() const { return size_; }
+
+ private:
+  std::map<size_t, size_t> free_;
+  size_t size_ = 0, used_ = 0;
+};
+
+struct Request {
+  long long id = 0;
+  ma_options opt{};
+  ma_status status = MA_OK;
+  std::string error;
+  double t_submit = 0.0, t_admit = 0.0, t_finish = 0.0;
+  long long cells = 0;
+  size_t bytes = 0, offset = 0;
+  bool has_region = false, finished = false;
+  ma_solver* solver = nullptr;
+  int total = 0, enqueued = 0, done = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int ev_steps[2] = {0, 0};
+  int head = 0, inflight = 0;
+  std::vector<double> U;
+};
+
+}  // namespace
+
+struct ma_server {
+  ma_server_config cfg{};
+  int device = 0;
+  char* base = nullptr;
+  Region region;
+  Clock::time_point t0;
+  std::mutex mu;
+  std::condition_variable cv_work;   // setup threads: a request was queued, or stop
+  std::condition_variable cv_admit;  // admission turn, slot or region changed
+  std::condition_variable cv_sched;  // scheduler: a case became active, or stop
+  std::condition_variable cv_done;   // pollers: a case finished
+  std::deque<Request*> queue;        // submitted, not yet taken by a setup thread
+  std::deque<Request*> incoming;     // built, not yet taken by the scheduler
+  std::deque<Request*> done_unpolled;
+  std::map<long long, std::unique_ptr<Request>> all;
+  long long next_id = 1;
+  long long admit_next = 1;  // admission follows submission order
+  long long waiting = 0;     // submitted, neither admitted nor finished
+  int active = 0;            // admitted, not finished
+  long long finished = 0;
+  bool stop = false;
+  std::vector<std::thread> setup;
+  std::thread sched;
+
+  double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+};
+
+namespace {
+
+// Under the lock: the request is done (successfully or not).  A case that
+// held a slot finished when it gave the slot back.
+void mark_finished(ma_server* S, Request* r) {
+  r->finished = true;
+  if (r->t_finish == 0.0) r->t_finish = S->now();
+  S->done_unpolled.push_back(r);
+  ++S->finished;
+  S->cv_done.notify_all();
+}
+
+void fail_request(ma_server* S, Request* r, ma_status st, const std::string& err) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  r->status = st;
+  r->error = err;
+  mark_finished(S, r);
+}
+
+// Frees a built case's device resources and returns its region block.
+void retire(ma_server* S, Request* r) {
+  if (r->solver) ma_solver_destroy(r->solver);
+  r->solver = nullptr;
+  for (auto& e : r->ev)
+    if (e) {
+      (void)hipEventDestroy(e);
+      e = nullptr;
+    }
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (r->has_region) {
+    S->region.release(r->offset, r->bytes);
+    r->has_region = false;
+    r->t_finish = S->now();
+    --S->active;
+    S->cv_admit.notify_all();
+  }
+}
+
+void setup_worker(ma_server* S) {
+  (void)hipSetDevice(S->device);
+  for (;;) {
+    Request* r = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty(); });
+      if (S->stop) return;  // queued requests are cancelled by ma_server_destroy
+      r = S->queue.front();
+      S->queue.pop_front();
+    }
+    // plan on this thread, outside the lock
+    ma_mesh* mesh = nullptr;
+    ma::PlannedCase* pc = nullptr;
+    size_t bytes = 0;
+    std::string err;
+    ma_status st = ma_mesh_generate(&r->opt, &mesh);
+    if (!st) {
+      ma_mesh_data md;
+      st = ma_mesh_view(mesh, &md);
+      if (!st) {
+        r->cells = md.num_owned_cells;
+        st = ma::plan_case(&md, &r->opt, &pc, &bytes);
+      }
+    }
+    if (st) err = ma::last_error();
+    if (mesh) ma_mesh_destroy(mesh);  // the plan holds its own copies
+
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv_admit.wait(lk, [&] { return S->stop || S->admit_next == r->id; });
+      if (!st && !S->stop && Region::granule(bytes) > S->region.size()) {
+        st = MA_ERR_NOMEM;
+        err = "case needs " + std::to_string(bytes) + " device bytes, more than the server's region";
+      }
+      if (!st)
+        S->cv_admit.wait(lk, [&] {
+          return S->stop || (S->active < S->cfg.max_active && S->region.reserve(bytes, &r->offset));
+        });
+      if (!st && S->stop) {
+        st = MA_ERR_INVALID;
+        err = "server stopped before the case was admitted";
+      }
+      if (!st) {
+        r->has_region = true;
+        r->bytes = bytes;
+        r->t_admit = S->now();
+        ++S->active;
+      }
+      --S->waiting;
+      ++S->admit_next;
+      S->cv_admit.notify_all();
+    }
+    if (st) {
+      ma::destroy_plan(pc);
+      fail_request(S, r, st, err);
+      continue;
+    }
+    // build inside the region, outside the lock
+    ma_solver* s = nullptr;
+    st = ma::create_from_plan(pc, S->base + r->offset, r->bytes, &s);
+    ma::destroy_plan(pc);
+    r->solver = s;
+    if (!st) st = ma_solver_initialize(s);
+    if (!st) st = ma_solver_set_graph(s, S->cfg.steps_per_launch);
+    for (auto& e : r->ev)
+      if (!st && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+        st = ma::fail(MA_ERR_DEVICE, "hipEventCreate failed");
+    if (st) {
+      err = ma::last_error();
+      retire(S, r);
+      fail_request(S, r, st, err);
+      continue;
+    }
+    r->total = r->opt.ntimesteps;
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->incoming.push_back(r);
+    }
+    S->cv_sched.notify_one();
+  }
+}
+
+// Copies the final state to the host, frees the case's device resources.
+void complete(ma_server* S, Request* r) {
+  if (r->status == MA_OK) {
+    r->U.resize((size_t)r->cells * 5);
+    ma_status st = ma_solver_get_state(r->solver, r->U.data());
+    if (st) {
+      r->status = st;
+      r->error = ma::last_error();
+      r->U.clear();
+    }
+  }
+  retire(S, r);
+  std::lock_guard<std::mutex> lk(S->mu);
+  mark_finished(S, r);
+}
+
+void scheduler(ma_server* S) {
+  (void)hipSetDevice(S->device);
+  std::vector<Request*> act;
+  for (;;) {
+    bool stopping = false;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      if (act.empty()) S->cv_sched.wait(lk, [&] { return S->stop || !S->incoming.empty(); });
+      while (!S->incoming.empty()) {
+        act.push_back(S->incoming.front());
+        S->incoming.pop_front();
+      }
+      stopping = S->stop;
+    }
+    if (stopping) {
+      // launched work runs out; the cases are dropped
+      for (Request* r : act) {
+        (void)hipStreamSynchronize(ma::solver_stream(r->solver));
+        r->status = MA_ERR_INVALID;
+        r->error = "server stopped before the case finished";
+        retire(S, r);
+        std::lock_guard<std::mutex> lk(S->mu);
+        mark_finished(S, r);
+      }
+      return;
+    }
+    bool progress = false;
+    for (Request* r : act) {
+      while (r->inflight > 0) {  // retire finished launches
+        const hipError_t q = hipEventQuery(r->ev[r->head]);
+        if (q == hipErrorNotReady) break;
+        if (q != hipSuccess) {
+          r->status = MA_ERR_DEVICE;
+          r->error = std::string("case stream: ") + hipGetErrorString(q);
+          (void)hipStreamSynchronize(ma::solver_stream(r->solver));
+          r->inflight = 0;
+          break;
+        }
+        r->done += r->ev_steps[r->head];
+        r->head ^= 1;
+        --r->inflight;
+        progress = true;
+      }
+      while (r->status == MA_OK && r->inflight < 2 && r->enqueued < r->total) {  // keep it fed
+        const int n = std::min(S->cfg.steps_per_launch, r->total - r->enqueued);
+        const int slot = (r->head + r->inflight) & 1;
+        ma_status st = ma::solver_advance(r->solver, n);
+        if (!st && hipEventRecord(r->ev[slot], ma::solver_stream(r->solver)) != hipSuccess)
+          st = ma::fail(MA_ERR_DEVICE, "hipEventRecord failed");
+        if (st) {
+          r->status = st;
+          r->error = ma::last_error();
+          (void)hipStreamSynchronize(ma::solver_stream(r->solver));
+          r->inflight = 0;
+          break;
+        }
+        r->ev_steps[slot] = n;
+        ++r->inflight;
+        r->enqueued += n;
+        progress = true;
+      }
+      if (r->inflight == 0 && (r->status != MA_OK || r->done >= r->total)) {
+        complete(S, r);
+        progress = true;
+      }
+    }
+    act.erase(std::remove_if(act.begin(), act.end(), [](Request* r) { return r->finished; }), act.end());
+    if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(50));
+  }
+}
+
+}  // namespace
+
+extern "C" ma_status ma_server_create(const ma_server_config* cfg, ma_server** out) {
+  if (!out) return ma::fail(MA_ERR_INVALID, "ma_server_create: null argument");
+  *out = nullptr;
+  ma_server_config c = cfg ? *cfg : ma_server_config{};
+  if (c.hbm_budget_bytes < 0 || c.max_active < 0 || c.setup_threads < 0 || c.steps_per_launch < 0 ||
+      (c.steps_per_launch & 1))
+    return ma::fail(MA_ERR_INVALID, "ma_server_create: negative setting or odd steps_per_launch");
+  if (c.hbm_budget_bytes == 0) c.hbm_budget_bytes = 200000000000ll;
+  if (c.max_active == 0) c.max_active = 16;
+  if (c.setup_threads == 0) c.setup_threads = 4;
+  if (c.steps_per_launch == 0) c.steps_per_launch = 10;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return ma::fail(MA_ERR_DEVICE, "no HIP device available");
+  size_t freeb = 0, totalb = 0;
+  if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return ma::fail(MA_ERR_DEVICE, "hipMemGetInfo failed");
+  // leave room for the runtime, streams and graphs of the active cases
+  const size_t margin = std::max<size_t>(size_t(2) << 30, freeb / 50);
+  size_t size = std::min<size_t>((size_t)c.hbm_budget_bytes, freeb > margin ? freeb - margin : 0);
+  size = size / (size_t(2) << 20) * (size_t(2) << 20);
+  if (size < (size_t(64) << 20)) return ma::fail(MA_ERR_NOMEM, "ma_server_create: less than 64 MiB of HBM free");
+  ma_server* S = new (std::nothrow) ma_server();
+  if (!S) return ma::fail(MA_ERR_NOMEM, "ma_server_create: out of host memory");
+  S->cfg = c;
+  (void)hipGetDevice(&S->device);
+  if (hipMalloc((void**)&S->base, size) != hipSuccess) {
+    delete S;
+    return ma::fail(MA_ERR_NOMEM, "ma_server_create: hipMalloc of the case region failed");
+  }
+  S->region.reset(size);
+  S->t0 = Clock::now();
+  try {
+    for (int i = 0; i < c.setup_threads; ++i) S->setup.emplace_back(setup_worker, S);
+    S->sched = std::thread(scheduler, S);
+  } catch (const std::exception&) {
+    ma_server_destroy(S);
+    return ma::fail(MA_ERR_DEVICE, "ma_server_create: could not start threads");
+  }
+  *out = S;
+  return MA_OK;
+}
+
+extern "C" ma_status ma_server_submit(ma_server* S, const ma_options* opts, long long* id) {
+  if (!S || !opts || !id) return ma::fail(MA_ERR_INVALID, "ma_server_submit: null argument");
+  if (opts->ntimesteps < 0) return ma::fail(MA_ERR_INVALID, "ma_server_submit: negative ntimesteps");
+  std::unique_ptr<Request> r(new (std::nothrow) Request());
+  if (!r) return ma::fail(MA_ERR_NOMEM, "ma_server_submit: out of host memory");
+  r->opt = *opts;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->stop) return ma::fail(MA_ERR_INVALID, "ma_server_submit: server is stopping");
+    r->id = S->next_id++;
+    r->t_submit = S->now();
+    *id = r->id;
+    S->queue.push_back(r.get());
+    ++S->waiting;
+    S->all[r->id] = std::move(r);
+  }
+  S->cv_work.notify_one();
+  return MA_OK;
+}
+
+extern "C" int ma_server_poll(ma_server* S, ma_case_result* out, int max, double timeout_s) {
+  if (!S || (!out && max > 0) || max < 0) {
+    ma::fail(MA_ERR_INVALID, "ma_server_poll: bad argument");
+    return -1;
+  }
+  std::unique_lock<std::mutex> lk(S->mu);
+  if (S->done_unpolled.empty() && timeout_s > 0.0)
+    S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s),
+                        [&] { return !S->done_unpolled.empty() || S->stop; });
+  int n = 0;
+  while (n < max && !S->done_unpolled.empty()) {
+    Request* r = S->done_unpolled.front();
+    S->done_unpolled.pop_front();
+    ma_case_result& o = out[n++];
+    o.id = r->id;
+    o.status = r->status;
+    o.steps = r->done;
+    o.cells = r->cells;
+    o.device_bytes = (long long)r->bytes;
+    o.t_submit = r->t_submit;
+    o.t_admit = r->t_admit;
+    o.t_finish = r->t_finish;
+  }
+  return n;
+}
+
+extern "C" ma_status ma_server_get_state(ma_server* S, long long id, double* U) {
+  if (!S || !U) return ma::fail(MA_ERR_INVALID, "ma_server_get_state: null argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->all.find(id);
+  if (it == S->all.end()) return ma::fail(MA_ERR_INVALID, "ma_server_get_state: unknown or released id");
+  Request* r = it->second.get();
+  if (!r->finished) return ma::fail(MA_ERR_INVALID, "ma_server_get_state: case not finished");
+  if (r->status != MA_OK || r->U.empty())
+    return ma::fail(r->status ? (ma_status)r->status : MA_ERR_INVALID, "case failed: " + r->error);
+  std::memcpy(U, r->U.data(), r->U.size() * sizeof(double));
+  return MA_OK;
+}
+
+extern "C" ma_status ma_server_release(ma_server* S, long long id) {
+  if (!S) return ma::fail(MA_ERR_INVALID, "ma_server_release: null server");
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->all.find(id);
+  if (it == S->all.end()) return ma::fail(MA_ERR_INVALID, "ma_server_release: unknown or released id");
+  if (!it->second->finished) return ma::fail(MA_ERR_INVALID, "ma_server_release: case not finished");
+  S->done_unpolled.erase(std::remove(S->done_unpolled.begin(), S->done_unpolled.end(), it->second.get()),
+                         S->done_unpolled.end());
+  S->all.erase(it);
+  return MA_OK;
+}
+
+extern "C" ma_status ma_server_stats(ma_server* S, long long* stats) {
+  if (!S || !stats) return ma::fail(MA_ERR_INVALID, "ma_server_stats: null argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  stats[0] = S->next_id - 1;
+  stats[1] = S->waiting;
+  stats[2] = S->active;
+  stats[3] = S->finished;
+  stats[4] = (long long)S->region.used();
+  stats[5] = (long long)S->region.size();
+  return MA_OK;
+}
+
+extern "C" double ma_server_now(ma_server* S) { return S ? S->now() : 0.0; }
+
+extern "C" void ma_server_destroy(ma_server* S) {
+  if (!S) return;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+  }
+  S->cv_work.notify_all();
+  S->cv_admit.notify_all();
+  S->cv_sched.notify_all();
+  S->cv_done.notify_all();
+  for (auto& t : S->setup)
+    if (t.joinable()) t.join();
+  if (S->sched.joinable()) S->sched.join();
+  // built after the scheduler stopped
+  for (Request* r : S->incoming) retire(S, r);
+  S->incoming.clear();
+  if (S->base) (void)hipFree(S->base);
+  delete S;
+}

@@ -737,9 +737,25 @@ struct ma_solver {
   ma_status alloc(T** p, size_t n) {
     return alloc_bytes((void**)p, n * sizeof(T));
   }
+  // Device memory: from hipMalloc, or carved from a caller-provided region
+  // (arena_base, 256-byte aligned), or only counted (counting: the pointers
+  // are placeholders and nothing is copied).
+  bool counting = false;
+  char* arena_base = nullptr;
+  size_t arena_size = 0, arena_used = 0;
+
   ma_status alloc_bytes(void** p, size_t nbytes) {
     *p = nullptr;
     if (nbytes == 0) nbytes = 8;
+    if (counting || arena_base) {
+      const size_t at = (arena_used + 255) & ~size_t(255);
+      if (!counting && at + nbytes > arena_size)
+        return ma::fail(MA_ERR_NOMEM, "device region too small for the case");
+      *p = counting ? reinterpret_cast<void*>(at + 256) : arena_base + at;
+      arena_used = at + nbytes;
+      bytes += nbytes;
+      return MA_OK;
+    }
     hipError_t e = hipMalloc(p, nbytes);
     if (e != hipSuccess)
       return ma::fail(MA_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
@@ -810,8 +826,9 @@ std::vector<int> processing_order(const ma_faces& F, int nc) {
 template <class T>
 ma_status upload(ma_solver* s, T** dst, const std::vector<T>& h) {
   ma_status st = s->alloc(dst, h.size());
-  if (st) return st;
-  MA_HIP(hipMemcpy(*dst, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  if (st || s->counting) return st;
+  MA_HIP(hipMemcpyAsync(*dst, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s->stream));
+  MA_HIP(hipStreamSynchronize(s->stream));
   return MA_OK;
 }
 
@@ -1340,21 +1357,25 @@ void plan_stats(const Layout& L, long long* stats) {
   stats[6] = L.derive_frame ? 1 : 0;
 }
 
-// Device arrays of one run: the layout, the per-case parameters and the work buffers.
-ma_status build(ma_solver* s, int ncases, const ma_mesh_data* const* mds) {
-  RunPlan P;
-  ma_status st = plan_run(ncases, mds, s->shape, P);
-  if (st) return st;
+// Device arrays of one run from its plan: the layout, the per-case
+// parameters and the work buffers.  With s->counting set nothing is
+// allocated or copied and the plan is left intact; s->arena_used is then the
+// device bytes the run needs.  Otherwise the plan's arrays are consumed.
+ma_status build_planned(ma_solver* s, RunPlan& P) {
+  ma_status st = MA_OK;
+  const int ncases = s->ncases;
   s->all_lattice = P.all_lattice;
   Layout& L = P.L;
   std::vector<int>& tile_case = P.tile_case;
   std::vector<int>& case_tiles = P.case_tiles;
   std::vector<int>& bc_case = P.bc_case;
   std::vector<int>& cell_case = P.cell_case;
-  s->case_cell_off.swap(P.case_cell_off);
   const int nc = L.nc, nif = L.nif, nf = L.nf;
   const long long nbf = L.nbf;
-  s->dev_of.swap(L.dev_of);
+  if (!s->counting) {
+    s->case_cell_off.swap(P.case_cell_off);
+    s->dev_of.swap(L.dev_of);
+  }
   DevMesh& m = s->m;
   m.nc = nc;
   m.nif = nif;
@@ -1393,7 +1414,7 @@ ma_status build(ma_solver* s, int ncases, const ma_mesh_data* const* mds) {
   s->t.slot = dtslot;
   s->t.cptr = dtcptr;
   s->t.tcase = dtcase;
-  L = Layout();
+  if (!s->counting) L = Layout();
 
   // per-case parameters
   std::vector<double> dt0(ncases), cfl(ncases), zero(ncases, 0.0);
@@ -1417,7 +1438,7 @@ ma_status build(ma_solver* s, int ncases, const ma_mesh_data* const* mds) {
     }
     p.perturb = o.perturb;
     p.seed = o.seed;
-    p.cell0 = s->case_cell_off[k];
+    p.cell0 = (s->counting ? P.case_cell_off : s->case_cell_off)[k];
   }
   if ((st = upload(s, &s->dt_dev, dt0)) || (st = upload(s, &s->cfl_dev, cfl)) ||
       (st = upload(s, &s->time_dev, zero)) || (st = upload(s, &s->ic_dev, ic)) ||
@@ -1433,10 +1454,19 @@ ma_status build(ma_solver* s, int ncases, const ma_mesh_data* const* mds) {
       (st = s->alloc_bytes(&s->bc, (size_t)5 * std::max<long long>(nbf, 1) * tsize)) ||
       (st = s->alloc(&s->partial, (size_t)std::max(1, s->t.ntiles))) || (st = s->alloc(&s->counter, 1)))
     return st;
-  MA_HIP(hipMemset(s->R, 0, n5 * sizeof(double)));
-  MA_HIP(hipMemset(s->grad, 0, 3 * n5 * tsize));
-  MA_HIP(hipMemset(s->phi, 0, n5 * tsize));
+  if (s->counting) return MA_OK;
+  MA_HIP(hipMemsetAsync(s->R, 0, n5 * sizeof(double), s->stream));
+  MA_HIP(hipMemsetAsync(s->grad, 0, 3 * n5 * tsize, s->stream));
+  MA_HIP(hipMemsetAsync(s->phi, 0, n5 * tsize, s->stream));
+  MA_HIP(hipStreamSynchronize(s->stream));
   return MA_OK;
+}
+
+ma_status build(ma_solver* s, int ncases, const ma_mesh_data* const* mds) {
+  RunPlan P;
+  ma_status st = plan_run(ncases, mds, s->shape, P);
+  if (st) return st;
+  return build_planned(s, P);
 }
 
 template <bool SO, class Visc, class Lim, class T>
@@ -1988,3 +2018,94 @@ extern "C" void ma_solver_destroy(ma_solver* s) {
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
+
+// ------------------------------------------- library-internal interface --
+// Used by the served form (server.hip): a single case planned on the host,
+// its exact device size known before any device memory is taken, then built
+// inside a region the caller hands over.
+
+namespace ma {
+
+struct PlannedCase {
+  RunPlan P;
+  ma_options opt;
+  TileShape shape;
+  size_t bytes = 0;
+};
+
+ma_status plan_case(const ma_mesh_data* md, const ma_options* opt, PlannedCase** out, size_t* bytes) {
+  *out = nullptr;
+  ma_status st = check_mesh(md, "serve");
+  if (st || (st = check_options(opt))) return st;
+  const TileShape shape = shape_of(*opt);
+  if (!valid_shape(shape)) return ma::fail(MA_ERR_INVALID, "tile shape: extents >= 1 with product <= 256");
+  PlannedCase* pc = new (std::nothrow) PlannedCase();
+  if (!pc) return ma::fail(MA_ERR_NOMEM, "serve: out of host memory");
+  try {
+    pc->opt = *opt;
+    pc->shape = shape;
+    st = plan_run(1, &md, shape, pc->P);
+    if (!st) {
+      ma_solver count;  // sizes only: no device memory, no stream
+      count.counting = true;
+      count.opt = *opt;
+      count.ncases = 1;
+      count.case_opt.assign(1, *opt);
+      count.fp32 = opt->precision == MA_PRECISION_FP32;
+      count.shape = shape;
+      st = build_planned(&count, pc->P);
+      pc->bytes = count.arena_used;
+    }
+  } catch (const std::bad_alloc&) {
+    st = ma::fail(MA_ERR_NOMEM, "serve: out of host memory");
+  }
+  if (st) {
+    delete pc;
+    return st;
+  }
+  *out = pc;
+  *bytes = pc->bytes;
+  return MA_OK;
+}
+
+void destroy_plan(PlannedCase* pc) { delete pc; }
+
+ma_status create_from_plan(PlannedCase* pc, void* region, size_t region_bytes, ma_solver** out) {
+  *out = nullptr;
+  ma_solver* s = new (std::nothrow) ma_solver();
+  if (!s) return ma::fail(MA_ERR_NOMEM, "serve: out of host memory");
+  ma_status st = MA_OK;
+  try {
+    s->shape = pc->shape;
+    s->opt = pc->opt;
+    s->ncases = 1;
+    s->case_opt.assign(1, pc->opt);
+    s->fp32 = pc->opt.precision == MA_PRECISION_FP32;
+    s->arena_base = static_cast<char*>(region);
+    s->arena_size = region_bytes;
+    (void)hipGetDevice(&s->device);
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+      st = ma::fail(MA_ERR_DEVICE, "hipStreamCreate failed");
+    else
+      st = build_planned(s, pc->P);
+  } catch (const std::bad_alloc&) {
+    st = ma::fail(MA_ERR_NOMEM, "serve: out of host memory");
+  }
+  if (st) {
+    ma_solver_destroy(s);
+    return st;
+  }
+  *out = s;
+  return MA_OK;
+}
+
+ma_status solver_advance(ma_solver* s, int nsteps) {
+  ma_status st = advance(s, nsteps);
+  if (st) return st;
+  MA_HIP(hipGetLastError());
+  return MA_OK;
+}
+
+hipStream_t solver_stream(const ma_solver* s) { return s->stream; }
+
+}  // namespace ma

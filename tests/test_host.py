@@ -17,7 +17,7 @@ from oracle import oracle as O
 def test_library_exports_every_declared_symbol():
     L = _lib.load()
     header = open(os.path.join(ROOT, "include", "miniaero.h")).read()
-    declared = set(re.findall(r"^(?:const char\*|ma_status|void|int)\s+(ma_[a-z_]+)\s*\(",
+    declared = set(re.findall(r"^(?:const char\*|ma_status|void|int|double)\s+(ma_[a-z_]+)\s*\(",
                               header, re.M))
     assert declared == set(_lib.EXPORTS)
     for name in declared:
@@ -249,3 +249,31 @@ def test_invalid_tile_shape_is_rejected_before_device_use():
         with pytest.raises(M.MiniAeroError) as e:
             M.TimeSolverExplicitRK4(md, M.Options(dt=1e-6, tile_shape=shape))
         assert e.value.status == _lib.MA_ERR_INVALID
+
+
+def test_server_checks_config_and_device_cleanly():
+    with pytest.raises(M.MiniAeroError) as e:
+        M.Server(steps_per_launch=3)
+    assert e.value.status == _lib.MA_ERR_INVALID
+    with pytest.raises(M.MiniAeroError) as e:
+        M.Server(max_active=-1)
+    assert e.value.status == _lib.MA_ERR_INVALID
+    if not os.path.exists("/dev/kfd"):
+        with pytest.raises(M.MiniAeroError) as e:
+            M.Server()
+        assert e.value.status == _lib.MA_ERR_DEVICE
+
+
+def test_served_stream_is_seeded_and_shaped():
+    from miniaero_amd.configs import served_c5
+    a, b = served_c5(300), served_c5(300)
+    assert [(t, o) for t, o in a] == [(t, o) for t, o in b]
+    t = np.array([x for x, _ in a])
+    assert np.all(np.diff(t) > 0) and 0.5 < t[-1] / 300 * 200 < 2.0   # ~200 arrivals per second
+    cells = np.array([o.nx * o.ny * o.nz for _, o in a])
+    assert cells.min() >= 4096 and cells.max() <= 2097152
+    assert all(o.nx == 4 * o.nz and o.ny == 2 * o.nz and o.precision == 1 for _, o in a)
+    visc = np.array([o.viscous for _, o in a])
+    assert 0.35 < visc.mean() < 0.65
+    steps = np.array([o.ntimesteps for _, o in a])
+    assert steps.min() >= 50 and steps.max() <= 500
