@@ -175,9 +175,9 @@ def bench_single(args, M, opts, copy_gbs) -> dict:
     print(f"[bench] {args.workload}: {nc} cells, {nif} internal + {nbf} boundary faces, "
           f"setup {setup_s:.1f} s, device {sizes['device_bytes'] / 2**30:.2f} GiB", file=sys.stderr,
           flush=True)
-    shape, tuned_ms = solver.tile_shape()
-    print(f"[bench] tile shape {shape}" + (f" (autotuned, {tuned_ms:.3f} ms/step/Mcell)" if tuned_ms else ""),
-          file=sys.stderr, flush=True)
+    shape, wg, tuned_ms = solver.tuning()
+    print(f"[bench] tile shape {shape}, workgroups per CU {wg or 'all tiles'}"
+          + (f" (autotuned, {tuned_ms:.3f} ms/step/Mcell)" if tuned_ms else ""), file=sys.stderr, flush=True)
     elapsed = timed_run(solver, args.steps, args.warmup, args.graph)
     if solver.nonfinite():
         raise RuntimeError("non-finite state after the timed steps")
@@ -215,7 +215,8 @@ def bench_single(args, M, opts, copy_gbs) -> dict:
               "internal_faces": nif, "boundary_faces": nbf, "physics": physics,
               "cfl": opts.cfl, "last_dt": dt_last,
               "precision": "fp64" if opts.precision == 0 else "fp32 flux/gradient/limiter, fp64 state",
-              "tile_shape": list(shape), "autotuned_ms_per_step_per_Mcell": tuned_ms or None,
+              "tile_shape": list(shape), "workgroups_per_cu": wg,
+              "autotuned_ms_per_step_per_Mcell": tuned_ms or None,
               "graph_steps": args.graph}
     return dict(value=nc * 4 * args.steps / elapsed, elapsed=elapsed, config=config, roofline=roof,
                 cpu_baseline=cpu, fp32=opts.precision != 0, extra=extra)
@@ -234,9 +235,9 @@ def bench_batch(args, M, cases, copy_gbs) -> dict:
     print(f"[bench] {args.workload}: {len(cases)} cases, {nc} cells, {nif} internal + {nbf} boundary "
           f"faces, mesh generation {gen_s:.1f} s, setup {setup_s:.1f} s, device "
           f"{sizes['device_bytes'] / 2**30:.2f} GiB", file=sys.stderr, flush=True)
-    shape, tuned_ms = solver.tile_shape()
-    print(f"[bench] tile shape {shape}" + (f" (autotuned, {tuned_ms:.3f} ms/step/Mcell)" if tuned_ms else ""),
-          file=sys.stderr, flush=True)
+    shape, wg, tuned_ms = solver.tuning()
+    print(f"[bench] tile shape {shape}, workgroups per CU {wg or 'all tiles'}"
+          + (f" (autotuned, {tuned_ms:.3f} ms/step/Mcell)" if tuned_ms else ""), file=sys.stderr, flush=True)
     elapsed = timed_run(solver, args.steps, args.warmup, args.graph)
     if solver.nonfinite():
         raise RuntimeError("non-finite state after the timed steps")
@@ -277,7 +278,8 @@ def bench_batch(args, M, cases, copy_gbs) -> dict:
               "physics": "inviscid Roe, Green-Gauss, Van Albada, 2nd order, Sod-type states",
               "dt": cases[0].dt, "steps_per_case": cases[0].ntimesteps,
               "precision": "fp32 flux/gradient/limiter, fp64 state" if fp32 else "fp64",
-              "tile_shape": list(shape), "autotuned_ms_per_step_per_Mcell": tuned_ms or None,
+              "tile_shape": list(shape), "workgroups_per_cu": wg,
+              "autotuned_ms_per_step_per_Mcell": tuned_ms or None,
               "graph_steps": args.graph,
               "case_latency_ms": {"p50": round(per_case_ms, 2), "p99": round(per_case_ms, 2),
                                   "note": "all cases run in the same launches and finish together"}}
@@ -400,6 +402,8 @@ def main() -> int:
     ap.add_argument("--cases", type=int, default=200, help="served workload: requests in the stream")
     ap.add_argument("--max-active", type=int, default=0, help="served workload: cases on the GPU at once")
     ap.add_argument("--hbm-budget", type=float, default=200e9, help="served workload: device bytes for cases")
+    ap.add_argument("--wg-per-cu", type=int, default=None,
+                    help="resident workgroups per CU for the gradient/flux kernels (0: one per tile)")
     ap.add_argument("--copy-ceiling", action="store_true",
                     help="only measure the plain copy kernel (1 GiB, read + write) and exit")
     args = ap.parse_args()
@@ -430,6 +434,8 @@ def main() -> int:
                 cases = [o.replace(precision=args.precision) for o in cases]
             if args.autotune is not None:
                 cases = [o.replace(autotune=args.autotune) for o in cases]
+            if args.wg_per_cu is not None:
+                cases = [o.replace(workgroups_per_cu=args.wg_per_cu) for o in cases]
             r = bench_batch(args, M, cases, copy_gbs)
         else:
             opts = WORKLOADS[args.workload]()
@@ -437,6 +443,8 @@ def main() -> int:
                 opts = opts.replace(precision=args.precision)
             if args.autotune is not None:
                 opts = opts.replace(autotune=args.autotune)
+            if args.wg_per_cu is not None:
+                opts = opts.replace(workgroups_per_cu=args.wg_per_cu)
             r = bench_single(args, M, opts, copy_gbs)
     except RuntimeError as e:
         print(f"[bench] {e}", file=sys.stderr)

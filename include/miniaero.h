@@ -80,7 +80,9 @@ typedef struct ma_options {
    * 256 consecutive cells of the caller's order.  Meshes that are not a
    * lattice always use consecutive cells.  Results do not depend on it. */
   int tile_shape[3];
-  int autotune;              /* 1: time candidate tile shapes at create, keep the fastest */
+  int autotune;              /* 1: time candidate tile shapes and workgroups_per_cu at create, keep the fastest */
+  int workgroups_per_cu;     /* gradient and flux kernels: resident workgroups per CU walking the
+                              * tiles (0 = one workgroup per tile); results do not depend on it */
 } ma_options;
 
 /* One Faces<Device> (Faces.h:42-71), as host arrays. */
@@ -165,10 +167,11 @@ ma_status ma_solver_synchronize(ma_solver* s);
  * default) launches every kernel separately.  Same results either way.
  * No reference counterpart. */
 ma_status ma_solver_set_graph(ma_solver* s, int steps_per_graph);
-/* Tile shape in use (-1 -1 -1 for consecutive cells) and, after autotuning,
- * the measured ms per step per million cells of that shape (else 0).
- * No reference counterpart. */
-ma_status ma_solver_tile_shape(ma_solver* s, int* shape, double* tuned_ms);
+/* Launch shape in use: tile shape (-1 -1 -1 for consecutive cells),
+ * resident workgroups per CU (0 = one per tile) and, after autotuning, the
+ * measured ms per step per million cells of that choice (else 0).  Any
+ * argument may be NULL.  No reference counterpart. */
+ma_status ma_solver_tuning(ma_solver* s, int* tile_shape, int* workgroups_per_cu, double* tuned_ms);
 /* Copies the current solution (sol_n) to host, num_owned_cells x 5. */
 ma_status ma_solver_get_state(ma_solver* s, double* U);
 /* Residual of the last stage, num_owned_cells x 5. */
@@ -177,6 +180,12 @@ ma_status ma_solver_get_residual(ma_solver* s, double* R);
 ma_status ma_solver_get_gradients(ma_solver* s, double* grad, double* lim);
 /* Simulated time and the dt of the last step. */
 ma_status ma_solver_get_time(ma_solver* s, double* time, double* last_dt);
+/* L2 norms of the residual of the last RK stage, per case and conserved
+ * variable: norms[5*k + v] = sqrt(sum over case k's cells of R_v^2), with
+ * R = dt / vol * (sum of face fluxes) as in ma_solver_get_residual.  Summed on
+ * the device in a fixed order (repeatable bit for bit).  No reference
+ * counterpart (a convergence monitor). */
+ma_status ma_solver_residual_norms(ma_solver* s, double* norms);
 /* Sets *nonfinite to the number of non-finite values in the state. */
 ma_status ma_solver_check_finite(ma_solver* s, long long* nonfinite);
 /* The HIP stream (hipStream_t) all of the solver's work is queued on. */

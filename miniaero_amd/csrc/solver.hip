@@ -241,119 +241,116 @@ __global__ __launch_bounds__(kBlock) void k_prim(int nc, const double* __restric
 // Green-Gauss gradient (GreenGauss.h:51-270), then, for second order, the
 // stencil min/max (StencilLimiter.h:56-282) and the limiter
 // (StencilLimiter.h:288-500) of one cell, all from the cell's slot list.
+// The value unchanged, but opaque to the optimiser: loads addressed through
+// it inside a loop are not hoisted out (which would keep them all live).
+__device__ __forceinline__ int opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 template <bool kSecondOrder, class Limiter, class T>
-__global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const T* __restrict__ V,
-                                                 T* __restrict__ grad, T* __restrict__ phi,
-                                                 int per_xcd) {
-  // XCD-aware: each XCD takes a contiguous run of blocks (see k_flux)
-  const int b = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
-  const int c = b * kBlock + threadIdx.x;
-  if (c >= m.nc) return;
+__device__ __forceinline__ void grad_cell(const DevMesh& m, const T* __restrict__ V, T* __restrict__ grad,
+                                          T* __restrict__ phi, int c) {
   const int nc = m.nc, nf = m.nf;
-  T vc[5];
-  load5(V, nc, c, vc);
   const T vol = T(m.vol[c]);
-  int ent[kMaxSlots];
-#pragma unroll
-  for (int s = 0; s < kMaxSlots; ++s) ent[s] = m.slot[(size_t)s * nc + c];
-
-  T g[5][3];
-#pragma unroll
-  for (int k = 0; k < 5; ++k)
-#pragma unroll
-    for (int d = 0; d < 3; ++d) g[k][d] = T(0.0);
-  T mn[5], mx[5];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    mn[k] = T(1.0e300);
-    mx[k] = -T(1.0e300);
-  }
-
+  int ent[kMaxSlots], nbr[kMaxSlots];  // slot entries; neighbour cell (-1: boundary face)
 #pragma unroll
   for (int s = 0; s < kMaxSlots; ++s) {
-    const int e = ent[s];
-    if (e < 0) break;
-    const int f = e >> 1;
-    T n[3];
-    load3(m.fn, nf, f, n);
-    if (f < m.nif) {
-      const int side = e & 1;
-      const int nb = side ? m.fl[f] : m.fr[f];
-      T vn[5];
-      load5(V, nc, nb, vn);
-      const T* vl = side ? vn : vc;
-      const T* vr = side ? vc : vn;
+    ent[s] = m.slot[(size_t)s * nc + c];
+    nbr[s] = -1;
+    const int f = ent[s] >> 1;
+    if (ent[s] >= 0 && f < m.nif) nbr[s] = (ent[s] & 1) ? m.fl[f] : m.fr[f];
+  }
+  double xcc[3];
+  if (kSecondOrder) load3(m.xc, nc, c, xcc);
+  // One conserved variable at a time: the five gradients, min/max and
+  // limiters are independent, and each accumulates over the slots in the
+  // same order as a joint loop would, so the results are the same bits with
+  // far fewer live registers (more waves in flight to hide gather latency).
+#pragma unroll 1
+  for (int k = 0; k < 5; ++k) {
+    const T* Vk = V + (size_t)k * nc;
+    const T vck = Vk[c];
+    T g[3] = {T(0.0), T(0.0), T(0.0)};
+    T mn = T(1.0e300), mx = -T(1.0e300);
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
+    for (int s = 0; s < kMaxSlots; ++s) {
+      const int e = ent[s];
+      if (e < 0) break;
+      const int f = opaque(e >> 1);
+      T n[3];
+      load3(m.fn, nf, f, n);
+      if (nbr[s] >= 0) {
+        const int side = e & 1;
+        const T vn = Vk[nbr[s]];
+        const T vl = side ? vn : vck;
+        const T vr = side ? vck : vn;
 #pragma unroll
-        for (int k = 0; k < 5; ++k) {
-          const T gg = T(0.5) * (vl[k] + vr[k]) * n[d];
-          g[k][d] += side ? -gg / vol : gg / vol;
+        for (int d = 0; d < 3; ++d) {
+          const T gg = T(0.5) * (vl + vr) * n[d];
+          g[d] += side ? -gg / vol : gg / vol;
         }
-      if (kSecondOrder) {
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-          const T fmin = vr[k] < vl[k] ? vr[k] : vl[k];
-          const T fmax = vr[k] > vl[k] ? vr[k] : vl[k];
-          mn[k] = ma::smin(mn[k], fmin);
-          mx[k] = ma::smax(mx[k], fmax);
+        if (kSecondOrder) {
+          const T fmin = vr < vl ? vr : vl;
+          const T fmax = vr > vl ? vr : vl;
+          mn = ma::smin(mn, fmin);
+          mx = ma::smax(mx, fmax);
         }
-      }
-    } else {
+      } else {
 #pragma unroll
-      for (int k = 0; k < 5; ++k)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) g[k][d] += vc[k] * n[d] / vol;
-      if (kSecondOrder) {
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-          mn[k] = ma::smin(mn[k], vc[k]);
-          mx[k] = ma::smax(mx[k], vc[k]);
+        for (int d = 0; d < 3; ++d) g[d] += vck * n[d] / vol;
+        if (kSecondOrder) {
+          mn = ma::smin(mn, vck);
+          mx = ma::smax(mx, vck);
         }
       }
     }
-  }
-  // green_gauss_gradient_sum: gradient = 0; gradient += slot (GreenGauss.h:244-269)
-#pragma unroll
-  for (int k = 0; k < 5; ++k)
+    // green_gauss_gradient_sum: gradient = 0; gradient += slot (GreenGauss.h:244-269)
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       T x = 0;
-      x += g[k][d];
-      g[k][d] = x;
+      x += g[d];
+      g[d] = x;
       grad[(size_t)(k * 3 + d) * nc + c] = x;
     }
-  if (!kSecondOrder) return;
-
-  double xcc[3];
-  load3(m.xc, nc, c, xcc);
-  T lim[5];
+    if (!kSecondOrder) continue;
+    T lim = T(1.0);
 #pragma unroll
-  for (int k = 0; k < 5; ++k) lim[k] = T(1.0);
+    for (int s = 0; s < kMaxSlots; ++s) {
+      const int e = ent[s];
+      if (e < 0) break;
+      const int f = opaque(e >> 1);
+      // displacement from absolute FP64 coordinates, then the arithmetic type
+      T disp[3], dist = 0;
 #pragma unroll
-  for (int s = 0; s < kMaxSlots; ++s) {
-    const int e = ent[s];
-    if (e < 0) break;
-    const int f = e >> 1;
-    // displacement from absolute FP64 coordinates, then the arithmetic type
-    T disp[3], dist = 0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      disp[d] = T(m.fx[(size_t)d * nf + f] - xcc[d]);
-      dist += disp[d] * disp[d];
-    }
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
+      for (int d = 0; d < 3; ++d) {
+        disp[d] = T(m.fx[(size_t)d * nf + f] - xcc[d]);
+        dist += disp[d] * disp[d];
+      }
       T du = 0;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) du += disp[d] * g[k][d];
-      const T dumax = mx[k] - vc[k];
-      const T dumin = mn[k] - vc[k];
-      lim[k] = ma::smin(lim[k], Limiter::limit(dumax, dumin, du, dist));
+      for (int d = 0; d < 3; ++d) du += disp[d] * g[d];
+      const T dumax = mx - vck;
+      const T dumin = mn - vck;
+      lim = ma::smin(lim, Limiter::limit(dumax, dumin, du, dist));
     }
+    phi[(size_t)k * nc + c] = lim;
   }
-#pragma unroll
-  for (int k = 0; k < 5; ++k) phi[(size_t)k * nc + c] = lim[k];
+}
+
+// amdgpu_waves_per_eu(6): register budget for six waves per SIMD (the FP64
+// second-order form then needs no scratch; eight would spill).
+template <bool kSecondOrder, class Limiter, class T>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_grad(DevMesh m, const T* __restrict__ V,
+                                                 T* __restrict__ grad, T* __restrict__ phi,
+                                                 int per_xcd) {
+  // XCD-aware: each XCD takes a contiguous run of blocks, walked with stride
+  // gridDim.x / kXcds when the grid is smaller than the block count (see k_flux)
+  const int xcd = blockIdx.x % kXcds, stride = gridDim.x / kXcds;
+  for (int j = blockIdx.x / kXcds; j < per_xcd; j += stride) {
+    const int c = (xcd * per_xcd + j) * kBlock + threadIdx.x;
+    if (c < m.nc) grad_cell<kSecondOrder, Limiter, T>(m, V, grad, phi, c);
+  }
 }
 
 // Reconstructed (second order) primitive state of cell `c` at face centroid xf:
@@ -497,20 +494,11 @@ __global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const T* __restrict__ 
 // internal face (index = LDS slot), 4 + kind - 1 = boundary face of that kind
 // (index = boundary-face number).
 template <bool kSecondOrder, class Visc, class T>
-__global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const T* __restrict__ V,
-                                                const T* __restrict__ grad,
-                                                const T* __restrict__ phi,
-                                                const T* __restrict__ bc,
-                                                const double* Uacc_in,  // may alias Uacc_out
-                                                double* Uacc_out, double* __restrict__ R,
-                                                const double* __restrict__ dt_dev, double beta,
-                                                Visc visc) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  T* lds = reinterpret_cast<T*>(lds_raw);  // 5 planes of t.max_faces
-  // XCD-aware order: the workgroups an XCD receives (every kXcds-th) take a
-  // contiguous run of tiles, so neighbouring tiles share that XCD's L2.
-  const int tile = (blockIdx.x % kXcds) * t.per_xcd + blockIdx.x / kXcds;
-  if (tile >= t.ntiles) return;  // whole workgroup, before any barrier
+__device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, int tile, T* lds,
+                                          const T* __restrict__ V, const T* __restrict__ grad,
+                                          const T* __restrict__ phi, const T* __restrict__ bc,
+                                          const double* Uacc_in, double* Uacc_out, double* __restrict__ R,
+                                          const double* __restrict__ dt_dev, double beta, const Visc& visc) {
   const int f0 = t.ptr[tile];
   const int nfl = t.ptr[tile + 1] - f0;
   const int mf = t.max_faces;
@@ -522,7 +510,7 @@ __global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const T* _
   }
   __syncthreads();
   const int c = t.cptr[tile] + threadIdx.x;
-  if (c >= t.cptr[tile + 1]) return;
+  if (c >= t.cptr[tile + 1]) return;  // (the caller's barrier is reached by every thread)
   const int nc = m.nc;
   const int nbf = m.nf - m.nif;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -566,6 +554,35 @@ __global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const T* _
     r = r + dt / vol * acc[k];
     R[(size_t)k * nc + c] = r;
     Uacc_out[(size_t)k * nc + c] = Uacc_in[(size_t)k * nc + c] + beta * r;
+  }
+}
+
+// Register budget: seven waves per SIMD in the FP32-arithmetic form (no
+// scratch); the FP64 form keeps the compiler's choice (its LDS buffer allows
+// four workgroups per CU in any case).
+template <bool kSecondOrder, class Visc, class T>
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 7 : 1))) void k_flux(DevMesh m, TileMap t, const T* __restrict__ V,
+                                                const T* __restrict__ grad,
+                                                const T* __restrict__ phi,
+                                                const T* __restrict__ bc,
+                                                const double* Uacc_in,  // may alias Uacc_out
+                                                double* Uacc_out, double* __restrict__ R,
+                                                const double* __restrict__ dt_dev, double beta,
+                                                Visc visc) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* lds = reinterpret_cast<T*>(lds_raw);  // 5 planes of t.max_faces
+  // XCD-aware order: the workgroups an XCD receives (every kXcds-th) take a
+  // contiguous run of tiles, so neighbouring tiles share that XCD's L2.  With
+  // fewer workgroups than tiles (gridDim.x a multiple of kXcds) each one
+  // walks its XCD's run with stride gridDim.x / kXcds, which bounds how many
+  // tiles an XCD has in flight.
+  const int xcd = blockIdx.x % kXcds, stride = gridDim.x / kXcds;
+  for (int j = blockIdx.x / kXcds; j < t.per_xcd; j += stride) {
+    const int tile = xcd * t.per_xcd + j;
+    if (tile >= t.ntiles) break;  // uniform over the workgroup
+    flux_tile<kSecondOrder, Visc, T>(m, t, tile, lds, V, grad, phi, bc, Uacc_in, Uacc_out, R, dt_dev, beta,
+                                     visc);
+    __syncthreads();  // the LDS is reused by the next tile
   }
 }
 
@@ -648,6 +665,41 @@ __global__ __launch_bounds__(kBlock) void k_dt_final(const double* __restrict__ 
   }
 }
 
+// Residual norms: per tile, the five sums of R_k^2 over its cells (wavefront
+// shuffles, then the tile's waves in order); per case, the tiles' sums in
+// tile order and the square root.  Fixed order, so repeatable bit for bit.
+__global__ __launch_bounds__(kTile) void k_norm_partial(TileMap t, int nc, const double* __restrict__ R,
+                                                        double* __restrict__ partial) {
+  __shared__ double red[5][kTile / 64];
+  const int tile = blockIdx.x;
+  const int c = t.cptr[tile] + threadIdx.x;
+  const bool in = c < t.cptr[tile + 1];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const double r = in ? R[(size_t)k * nc + c] : 0.0;
+    double x = r * r;
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) red[k][wave] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    double y = 0.0;
+    for (int w = 0; w < kTile / 64; ++w) y += red[threadIdx.x][w];
+    partial[(size_t)tile * 5 + threadIdx.x] = y;
+  }
+}
+
+__global__ void k_norm_final(const double* __restrict__ partial, const int* __restrict__ case_tiles,
+                             int ncases, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncases * 5) return;
+  const int k = i / 5, v = i % 5;
+  double y = 0.0;
+  for (int b = case_tiles[k]; b < case_tiles[k + 1]; ++b) y += partial[(size_t)b * 5 + v];
+  out[i] = sqrt(y);
+}
+
 __global__ __launch_bounds__(kBlock) void k_count_nonfinite(int n, const double* __restrict__ a,
                                                             unsigned long long* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -722,6 +774,7 @@ struct ma_solver {
   IcParams* ic_dev = nullptr;
   Inflow* inflow_dev = nullptr;
   unsigned long long* counter = nullptr;
+  double *norm_partial = nullptr, *norm_out = nullptr;  // per tile x 5, per case x 5
   int cur = 0;  // U[cur] holds sol_n
   long long steps_done = 0;
   // HIP graphs of graph_steps whole RK4 steps (an even count, so U[cur] is
@@ -729,6 +782,8 @@ struct ma_solver {
   int graph_steps = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   TileShape shape = kDefaultShape;  // flux tile shape in use
+  int wg_per_cu = 0;                 // resident workgroups per CU of k_grad/k_flux (0: full grid)
+  int cus = 256;                     // compute units of the device
   bool all_lattice = true;
   double tuned_ms = 0.0;  // autotuner: measured ms per step per million cells of the chosen shape
   Timing timing;
@@ -881,9 +936,9 @@ bool lattice_coords(const ma_mesh_data* md, std::vector<int> ijk[3], int dims[3]
   return true;
 }
 
-// Device cell order: bricks of the tile shape's lattice cells
-// (brick-lexicographic, last axis fastest, and the same inside a brick), so a
-// tile is a compact block whose neighbours are mostly inside it.  Without a
+// Device cell order: bricks of the tile shape's lattice cells (cells inside a
+// brick lexicographic, last axis fastest), so a tile is a compact block whose
+// neighbours are mostly inside it.  Without a
 // lattice, or with the all-zero shape, the original order is kept.  Fills
 // dev_of (original -> device) and the cell range of every tile.
 void cell_order(const ma_mesh_data* md, const TileShape& shape, std::vector<int>& dev_of,
@@ -904,16 +959,43 @@ void cell_order(const ma_mesh_data* md, const TileShape& shape, std::vector<int>
   }
   std::vector<int> at((size_t)nc);
   for (int c = 0; c < nc; ++c) at[((size_t)ijk[0][c] * dims[1] + ijk[1][c]) * dims[2] + ijk[2][c]] = c;
+  // Brick order: the brick grid is cut into kXcds slabs along its longest
+  // axis (the kernels hand each XCD a contiguous run of tiles), and inside a
+  // slab the axis of smallest extent runs fastest, so a brick's neighbours on
+  // all three axes are close to it in dispatch order and still in that XCD's
+  // L2 when it runs.
+  int nb[3];
+  for (int a = 0; a < 3; ++a) nb[a] = (dims[a] + kBrick[a] - 1) / kBrick[a];
+  int sa = 0;
+  for (int a = 1; a < 3; ++a)
+    if (nb[a] > nb[sa]) sa = a;
+  int ext[3] = {nb[0], nb[1], nb[2]};
+  ext[sa] = (nb[sa] + kXcds - 1) / kXcds;
+  int ax[3] = {0, 1, 2};  // slowest .. fastest inside a slab
+  std::stable_sort(ax, ax + 3, [&](int a, int b) { return ext[a] > ext[b]; });
   int d = 0;
-  for (int bi = 0; bi < dims[0]; bi += kBrick[0])
-    for (int bj = 0; bj < dims[1]; bj += kBrick[1])
-      for (int bk = 0; bk < dims[2]; bk += kBrick[2]) {
-        for (int i = bi; i < std::min(bi + kBrick[0], dims[0]); ++i)
-          for (int j = bj; j < std::min(bj + kBrick[1], dims[1]); ++j)
-            for (int k = bk; k < std::min(bk + kBrick[2], dims[2]); ++k)
-              dev_of[at[((size_t)i * dims[1] + j) * dims[2] + k]] = d++;
-        tile_cptr.push_back(d);
-      }
+  for (int sl = 0; sl < kXcds; ++sl) {
+    const int lo = (int)((long long)sl * nb[sa] / kXcds), hi = (int)((long long)(sl + 1) * nb[sa] / kXcds);
+    if (lo >= hi) continue;
+    int len[3], base[3];
+    for (int a = 0; a < 3; ++a) {
+      len[a] = a == sa ? hi - lo : nb[a];
+      base[a] = a == sa ? lo : 0;
+    }
+    int q[3];
+    for (q[0] = 0; q[0] < len[ax[0]]; ++q[0])
+      for (q[1] = 0; q[1] < len[ax[1]]; ++q[1])
+        for (q[2] = 0; q[2] < len[ax[2]]; ++q[2]) {
+          int bb[3];
+          for (int r = 0; r < 3; ++r) bb[ax[r]] = base[ax[r]] + q[r];
+          const int bi = bb[0] * kBrick[0], bj = bb[1] * kBrick[1], bk = bb[2] * kBrick[2];
+          for (int i = bi; i < std::min(bi + kBrick[0], dims[0]); ++i)
+            for (int j = bj; j < std::min(bj + kBrick[1], dims[1]); ++j)
+              for (int k = bk; k < std::min(bk + kBrick[2], dims[2]); ++k)
+                dev_of[at[((size_t)i * dims[1] + j) * dims[2] + k]] = d++;
+          tile_cptr.push_back(d);
+        }
+  }
 }
 
 // Everything the device needs, computed on the host (no GPU involved).
@@ -1452,7 +1534,9 @@ ma_status build_planned(ma_solver* s, RunPlan& P) {
       (st = s->alloc_bytes(&s->V, n5 * tsize)) || (st = s->alloc_bytes(&s->grad, 3 * n5 * tsize)) ||
       (st = s->alloc_bytes(&s->phi, n5 * tsize)) ||
       (st = s->alloc_bytes(&s->bc, (size_t)5 * std::max<long long>(nbf, 1) * tsize)) ||
-      (st = s->alloc(&s->partial, (size_t)std::max(1, s->t.ntiles))) || (st = s->alloc(&s->counter, 1)))
+      (st = s->alloc(&s->partial, (size_t)std::max(1, s->t.ntiles))) || (st = s->alloc(&s->counter, 1)) ||
+      (st = s->alloc(&s->norm_partial, (size_t)5 * std::max(1, s->t.ntiles))) ||
+      (st = s->alloc(&s->norm_out, (size_t)5 * ncases)))
     return st;
   if (s->counting) return MA_OK;
   MA_HIP(hipMemsetAsync(s->R, 0, n5 * sizeof(double), s->stream));
@@ -1467,6 +1551,12 @@ ma_status build(ma_solver* s, int ncases, const ma_mesh_data* const* mds) {
   ma_status st = plan_run(ncases, mds, s->shape, P);
   if (st) return st;
   return build_planned(s, P);
+}
+
+// Workgroups per XCD for a kernel with `per` tiles or blocks per XCD.
+inline int resident(const ma_solver* s, int per) {
+  if (s->wg_per_cu <= 0) return per;
+  return std::max(1, std::min(per, s->wg_per_cu * std::max(1, s->cus / kXcds)));
 }
 
 template <bool SO, class Visc, class Lim, class T>
@@ -1485,7 +1575,7 @@ void run_stage(ma_solver* s, int irk) {
   if (SO || Visc::isViscous) {
     s->launch(SO ? "k_grad_limiter" : "k_grad", [&] {
       const int per = (nb + kXcds - 1) / kXcds;
-      hipLaunchKernelGGL((k_grad<SO, Lim, T>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, m,
+      hipLaunchKernelGGL((k_grad<SO, Lim, T>), dim3(resident(s, per) * kXcds), dim3(kBlock), 0, s->stream, m,
                          static_cast<const T*>(s->V), static_cast<T*>(s->grad), static_cast<T*>(s->phi),
                          per);
     });
@@ -1503,7 +1593,7 @@ void run_stage(ma_solver* s, int irk) {
   }
   const size_t lds = (size_t)s->t.max_faces * 5 * sizeof(T);
   s->launch("k_flux", [&] {
-    hipLaunchKernelGGL((k_flux<SO, Visc, T>), dim3(s->t.per_xcd * kXcds), dim3(kTile), lds, s->stream, m,
+    hipLaunchKernelGGL((k_flux<SO, Visc, T>), dim3(resident(s, s->t.per_xcd) * kXcds), dim3(kTile), lds, s->stream, m,
                        s->t, static_cast<const T*>(s->V), static_cast<const T*>(s->grad),
                        static_cast<const T*>(s->phi), static_cast<const T*>(s->bc), acc_in, Uacc, s->R,
                        s->dt_dev, beta[irk], visc);
@@ -1646,57 +1736,89 @@ TileShape shape_of(const ma_options& o) {
   return TileShape{{o.tile_shape[0], o.tile_shape[1], o.tile_shape[2]}};
 }
 
+// Launch-shape choices that do not change results: the flux tile shape and
+// how many workgroups per CU the gradient and flux kernels keep resident
+// (0 = one workgroup per tile or cell block).
+struct Tuning {
+  TileShape shape;
+  int wg_per_cu;
+};
+
 ma_status create_run(int ncases, const ma_mesh_data* const* mds, const ma_options* opts, ma_solver** out,
-                     const TileShape* force = nullptr);
+                     const Tuning* force = nullptr);
+
+// Times kTimed steps of a freshly built run (after kWarm untimed ones) from
+// its initial state; ms per step per million cells.
+ma_status time_candidate(int n, const ma_mesh_data* const* mds, const ma_options* so, const Tuning& c,
+                         long long cells, double* v, bool* lattice) {
+  constexpr int kWarm = 2, kTimed = 4;
+  ma_solver* t = nullptr;
+  ma_status st = create_run(n, mds, so, &t, &c);
+  if (st) return st;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float ms = 0.f;
+  bool ok = ma_solver_initialize(t) == MA_OK && hipEventCreate(&e0) == hipSuccess &&
+            hipEventCreate(&e1) == hipSuccess;
+  if (ok) {
+    dispatch(t, kWarm);
+    ok = hipEventRecord(e0, t->stream) == hipSuccess;
+    dispatch(t, kTimed);
+    ok = ok && hipEventRecord(e1, t->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+         hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+  }
+  *lattice = t->all_lattice;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  ma_solver_destroy(t);
+  if (!ok) return ma::fail(MA_ERR_DEVICE, "autotune: timing run failed");
+  *v = ms / kTimed / (cells * 1e-6);
+  return MA_OK;
+}
 
 // Autotuner: builds the leading cases (at least 2^21 cells, or all of them)
-// with every candidate tile shape, times a few steps of each from the
-// initial state with HIP events, and returns the fastest shape and its time
-// in ms per step per million cells.  Results do not depend on the shape
-// (every cell sums its faces in the same order whatever the tiles), only
-// the speed does.
-ma_status autotune(int ncases, const ma_mesh_data* const* mds, const ma_options* opts, TileShape* best,
+// with every candidate tile shape, then the best shape with several
+// resident-workgroup counts, times a few steps of each from the initial
+// state with HIP events, and returns the fastest choice and its time in ms
+// per step per million cells.  Results do not depend on these choices
+// (every cell sums its faces in the same order whatever the tiles and
+// however the workgroups walk them), only the speed does.
+ma_status autotune(int ncases, const ma_mesh_data* const* mds, const ma_options* opts, Tuning* best,
                    double* best_ms) {
   static const TileShape cand[] = {{{4, 8, 8}}, {{8, 8, 4}}, {{8, 4, 8}}, {{4, 4, 16}}, {{16, 4, 4}},
                                    {{2, 8, 16}}, {{8, 2, 16}}, {{4, 16, 4}}, {{2, 16, 8}}, {{4, 4, 8}},
                                    {{0, 0, 0}}};
-  constexpr int kWarm = 2, kTimed = 4;
+  static const int wgs[] = {1, 2, 3, 4};
   int n = 0;
   long long cells = 0;
   while (n < ncases && cells < (1ll << 21)) cells += mds[n++]->num_owned_cells;
   std::vector<ma_options> so(opts, opts + n);
   for (auto& o : so) o.autotune = 0;
   *best_ms = 0.0;
-  *best = kDefaultShape;
+  *best = Tuning{kDefaultShape, 0};
   double best_v = 1e300;
-  for (const TileShape& c : cand) {
-    ma_solver* t = nullptr;
-    ma_status st = create_run(n, mds, so.data(), &t, &c);
-    if (st == MA_ERR_UNSUPPORTED) continue;  // e.g. a tile with too many faces for LDS
+  auto consider = [&](const Tuning& c, bool* lattice) -> ma_status {
+    double v = 0.0;
+    ma_status st = time_candidate(n, mds, so.data(), c, cells, &v, lattice);
+    if (st == MA_ERR_UNSUPPORTED) return MA_OK;  // e.g. a tile with too many faces for LDS
     if (st) return st;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    float ms = 0.f;
-    bool ok = ma_solver_initialize(t) == MA_OK && hipEventCreate(&e0) == hipSuccess &&
-              hipEventCreate(&e1) == hipSuccess;
-    if (ok) {
-      dispatch(t, kWarm);
-      ok = hipEventRecord(e0, t->stream) == hipSuccess;
-      dispatch(t, kTimed);
-      ok = ok && hipEventRecord(e1, t->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
-           hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
-    }
-    const bool lattice = t->all_lattice;
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    ma_solver_destroy(t);
-    if (!ok) return ma::fail(MA_ERR_DEVICE, "autotune: timing run failed");
-    const double v = ms / kTimed / (cells * 1e-6);
     if (v < best_v) {
       best_v = v;
       *best = c;
       *best_ms = v;
     }
+    return MA_OK;
+  };
+  for (const TileShape& c : cand) {
+    bool lattice = true;
+    ma_status st = consider(Tuning{c, 0}, &lattice);
+    if (st) return st;
     if (!lattice) break;  // without a lattice every shape is the same layout
+  }
+  const TileShape shape = best->shape;
+  for (int w : wgs) {
+    bool lattice = true;
+    ma_status st = consider(Tuning{shape, w}, &lattice);
+    if (st) return st;
   }
   return MA_OK;
 }
@@ -1704,7 +1826,7 @@ ma_status autotune(int ncases, const ma_mesh_data* const* mds, const ma_options*
 // Shared body of ma_solver_create and ma_batch_create; `force` fixes the
 // tile shape (no autotuning).
 ma_status create_run(int ncases, const ma_mesh_data* const* mds, const ma_options* opts, ma_solver** out,
-                     const TileShape* force) {
+                     const Tuning* force) {
   const ma_options& o0 = opts[0];
   for (int k = 0; k < ncases; ++k) {
     ma_status st = check_mesh(mds[k], "create");
@@ -1716,8 +1838,10 @@ ma_status create_run(int ncases, const ma_mesh_data* const* mds, const ma_option
       return ma::fail(MA_ERR_INVALID,
                       "batched cases must share second_order_space, viscous, limiter, precision and prandtl");
   }
-  TileShape shape = force ? *force : shape_of(o0);
-  if (!valid_shape(shape)) return ma::fail(MA_ERR_INVALID, "tile shape: extents >= 1 with product <= 256");
+  Tuning tune = force ? *force : Tuning{shape_of(o0), o0.workgroups_per_cu};
+  if (!valid_shape(tune.shape)) return ma::fail(MA_ERR_INVALID, "tile shape: extents >= 1 with product <= 256");
+  if (tune.wg_per_cu < 0 || tune.wg_per_cu > 64)
+    return ma::fail(MA_ERR_INVALID, "workgroups_per_cu must be in 0..64");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return ma::fail(MA_ERR_DEVICE, "no HIP device available");
@@ -1725,7 +1849,7 @@ ma_status create_run(int ncases, const ma_mesh_data* const* mds, const ma_option
   if (!force && o0.autotune) {
     ma_status st = MA_OK;
     try {
-      st = autotune(ncases, mds, opts, &shape, &tuned_ms);
+      st = autotune(ncases, mds, opts, &tune, &tuned_ms);
     } catch (const std::bad_alloc&) {
       st = ma::fail(MA_ERR_NOMEM, "autotune: out of host memory");
     }
@@ -1735,13 +1859,15 @@ ma_status create_run(int ncases, const ma_mesh_data* const* mds, const ma_option
   if (!s) return ma::fail(MA_ERR_NOMEM, "create: out of host memory");
   ma_status st = MA_OK;
   try {
-    s->shape = shape;
+    s->shape = tune.shape;
+    s->wg_per_cu = tune.wg_per_cu;
     s->tuned_ms = tuned_ms;
     s->opt = o0;
     s->ncases = ncases;
     s->case_opt.assign(opts, opts + ncases);
     s->fp32 = o0.precision == MA_PRECISION_FP32;
     (void)hipGetDevice(&s->device);
+    (void)hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->device);
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
       st = ma::fail(MA_ERR_DEVICE, "hipStreamCreate failed");
     else
@@ -1808,10 +1934,12 @@ extern "C" ma_status ma_solver_step(ma_solver* s, int nsteps) {
   return MA_OK;
 }
 
-extern "C" ma_status ma_solver_tile_shape(ma_solver* s, int* shape, double* tuned_ms) {
-  if (!s || !shape) return ma::fail(MA_ERR_INVALID, "ma_solver_tile_shape: null argument");
+extern "C" ma_status ma_solver_tuning(ma_solver* s, int* tile_shape, int* workgroups_per_cu, double* tuned_ms) {
+  if (!s) return ma::fail(MA_ERR_INVALID, "ma_solver_tuning: null solver");
   const bool consecutive = s->shape.b[0] == 0;
-  for (int k = 0; k < 3; ++k) shape[k] = consecutive ? -1 : s->shape.b[k];
+  if (tile_shape)
+    for (int k = 0; k < 3; ++k) tile_shape[k] = consecutive ? -1 : s->shape.b[k];
+  if (workgroups_per_cu) *workgroups_per_cu = s->wg_per_cu;
   if (tuned_ms) *tuned_ms = s->tuned_ms;
   return MA_OK;
 }
@@ -1903,6 +2031,21 @@ extern "C" ma_status ma_batch_sizes(ma_solver* s, int* ncases, int* case_cell_of
   *ncases = s->ncases;
   if (case_cell_offsets)
     for (int k = 0; k <= s->ncases; ++k) case_cell_offsets[k] = s->case_cell_off[k];
+  return MA_OK;
+}
+
+extern "C" ma_status ma_solver_residual_norms(ma_solver* s, double* norms) {
+  if (!s || !norms) return ma::fail(MA_ERR_INVALID, "null argument");
+  if (s->t.ntiles > 0) {
+    hipLaunchKernelGGL(k_norm_partial, dim3(s->t.ntiles), dim3(kTile), 0, s->stream, s->t, s->m.nc, s->R,
+                       s->norm_partial);
+    hipLaunchKernelGGL(k_norm_final, dim3((5 * s->ncases + 255) / 256), dim3(256), 0, s->stream, s->norm_partial,
+                       s->case_tiles, s->ncases, s->norm_out);
+    MA_HIP(hipGetLastError());
+  }
+  MA_HIP(hipMemcpyAsync(norms, s->norm_out, (size_t)5 * s->ncases * sizeof(double), hipMemcpyDeviceToHost,
+                        s->stream));
+  MA_HIP(hipStreamSynchronize(s->stream));
   return MA_OK;
 }
 
@@ -2077,6 +2220,7 @@ ma_status create_from_plan(PlannedCase* pc, void* region, size_t region_bytes, m
   ma_status st = MA_OK;
   try {
     s->shape = pc->shape;
+    s->wg_per_cu = pc->opt.workgroups_per_cu;
     s->opt = pc->opt;
     s->ncases = 1;
     s->case_opt.assign(1, pc->opt);
@@ -2084,6 +2228,7 @@ ma_status create_from_plan(PlannedCase* pc, void* region, size_t region_bytes, m
     s->arena_base = static_cast<char*>(region);
     s->arena_size = region_bytes;
     (void)hipGetDevice(&s->device);
+    (void)hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->device);
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
       st = ma::fail(MA_ERR_DEVICE, "hipStreamCreate failed");
     else

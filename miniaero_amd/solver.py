@@ -66,12 +66,19 @@ class TimeSolverExplicitRK4:
     def step(self, n: int = 1) -> None:
         _lib.check(self._L.ma_solver_step(self._h, int(n)))
 
+    def tuning(self):
+        """(tile brick (i, j, k) in use, resident workgroups per CU, autotuned ms per
+        step per million cells or 0)."""
+        sh = (ctypes.c_int * 3)()
+        wg = ctypes.c_int()
+        ms = ctypes.c_double()
+        _lib.check(self._L.ma_solver_tuning(self._h, sh, ctypes.byref(wg), ctypes.byref(ms)))
+        return tuple(sh), wg.value, ms.value
+
     def tile_shape(self):
         """(tile brick (i, j, k) in use, autotuned ms per step per million cells or 0)."""
-        sh = (ctypes.c_int * 3)()
-        ms = ctypes.c_double()
-        _lib.check(self._L.ma_solver_tile_shape(self._h, sh, ctypes.byref(ms)))
-        return tuple(sh), ms.value
+        sh, _, ms = self.tuning()
+        return sh, ms
 
     def set_graph(self, steps_per_graph: int) -> None:
         """Run groups of steps_per_graph steps (even) as one captured HIP graph; 0 = off."""
@@ -101,6 +108,14 @@ class TimeSolverExplicitRK4:
         t, d = ctypes.c_double(), ctypes.c_double()
         _lib.check(self._L.ma_solver_get_time(self._h, ctypes.byref(t), ctypes.byref(d)))
         return t.value, d.value
+
+    def residual_norms(self) -> np.ndarray:
+        """L2 norm of the last stage's residual per conserved variable (one row
+        per case for a batch)."""
+        n = max(1, getattr(self, "ncases", 1))
+        out = np.empty((n, 5))
+        _lib.check(self._L.ma_solver_residual_norms(self._h, out.ctypes.data_as(_lib._D)))
+        return out[0] if not hasattr(self, "ncases") else out
 
     def nonfinite(self) -> int:
         n = ctypes.c_longlong(0)

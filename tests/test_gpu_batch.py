@@ -151,14 +151,18 @@ def test_graph_replays_match_plain_launches(graph_steps):
         b.close()
 
 
-@pytest.mark.parametrize("shape", [(-1, 0, 0), (1, 1, 1), (2, 8, 16), (8, 8, 4), (16, 4, 4), (3, 5, 7)])
-def test_tile_shape_does_not_change_results(shape):
+@pytest.mark.parametrize("shape,wg", [((-1, 0, 0), 0), ((1, 1, 1), 0), ((2, 8, 16), 1), ((8, 8, 4), 0),
+                                      ((16, 4, 4), 2), ((3, 5, 7), 0), ((4, 8, 8), 1), ((4, 8, 8), 3)])
+def test_tile_shape_does_not_change_results(shape, wg):
     """Every cell sums its faces in the same order whatever the flux tiles
-    are, so any tile shape gives the default layout's bits."""
-    kw = plate(nx=20, ny=18, nz=10, y_stretch=1.05, bc_kind=(3, 2, 2, 2, 1, 4), dt=0.0, cfl=0.5)
+    are and however many workgroups walk them, so any tile shape and any
+    resident-workgroup count give the default layout's bits."""
+    # 92k cells: more tiles per XCD than 32 CUs x 1 workgroup, so the
+    # workgroups of the capped grid walk several tiles each
+    kw = plate(nx=64, ny=40, nz=36, y_stretch=1.05, bc_kind=(3, 2, 2, 2, 1, 4), dt=0.0, cfl=0.5)
     md, opts = make_case(kw)
     out = []
-    for o in (opts, opts.replace(tile_shape=shape)):
+    for o in (opts, opts.replace(tile_shape=shape, workgroups_per_cu=wg)):
         s = M.TimeSolverExplicitRK4(md, o)
         try:
             s.initialize()
@@ -190,3 +194,19 @@ def test_autotuned_batch_matches_default():
     print(f"[gpu-batch] autotuned shape {shape}, {ms:.4f} ms/step/Mcell", flush=True)
     assert ms > 0.0
     assert np.array_equal(out[0], out[1])
+
+
+def test_batch_residual_norms_are_per_case():
+    cases = [make_case(kw) for kw in VISCOUS_CASES[:3]]
+    b = M.BatchSolver(cases)
+    try:
+        b.initialize()
+        b.step(4)
+        n = b.residual_norms()
+        R = b.residual()
+    finally:
+        b.close()
+    assert n.shape == (3, 5)
+    for k in range(3):
+        lo, hi = b.offsets[k], b.offsets[k + 1]
+        assert np.allclose(n[k], np.linalg.norm(R[lo:hi], axis=0), rtol=1e-13, atol=0.0)

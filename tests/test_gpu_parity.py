@@ -356,3 +356,31 @@ def test_stored_face_frames_match_derived_frames():
         out.append(s.state())
         s.close()
     assert np.array_equal(out[0], out[1])
+
+
+def test_residual_norms_reduce_the_residual():
+    """ma_solver_residual_norms against numpy norms of the oracle's residual
+    (bit-identical to the GPU's, so only the summation order differs:
+    relative 1e-13), repeatable bit for bit, and zero up to round-off for a
+    uniform flow with Extrapolate on every side (the closed form)."""
+    kw = dict(_case("FlatPlate_Serial"), ntimesteps=5)
+    p, _ = check_steps(kw, per_step=0, total=5)
+    try:
+        n = p.gpu.residual_norms()
+        ref = np.linalg.norm(p.cpu.residual(), axis=0)
+        _log(f"[gpu-parity] residual norms {n} vs oracle {ref}")
+        assert np.allclose(n, ref, rtol=1e-13, atol=0.0)
+        assert np.array_equal(n, p.gpu.residual_norms())
+    finally:
+        p.close()
+    kw = dict(CONFIG_CASES["plate_cfl_stretch"], bc_kind=(1, 1, 1, 1, 1, 1), viscous=0)
+    q = Pair(kw)
+    try:
+        U0 = q.gpu.state()
+        q.gpu.step(2)
+        n = q.gpu.residual_norms()
+        scale = np.linalg.norm(U0, axis=0)
+        _log(f"[gpu-parity] uniform-flow residual norms {n}, state norms {scale}")
+        assert (n <= 1e-13 * scale).all()
+    finally:
+        q.close()
