@@ -241,107 +241,118 @@ __global__ __launch_bounds__(kBlock) void k_prim(int nc, const double* __restric
 // Green-Gauss gradient (GreenGauss.h:51-270), then, for second order, the
 // stencil min/max (StencilLimiter.h:56-282) and the limiter
 // (StencilLimiter.h:288-500) of one cell, all from the cell's slot list.
-// The value unchanged, but opaque to the optimiser: loads addressed through
-// it inside a loop are not hoisted out (which would keep them all live).
-__device__ __forceinline__ int opaque(int x) {
-  asm volatile("" : "+v"(x));
-  return x;
-}
-
 template <bool kSecondOrder, class Limiter, class T>
 __device__ __forceinline__ void grad_cell(const DevMesh& m, const T* __restrict__ V, T* __restrict__ grad,
                                           T* __restrict__ phi, int c) {
   const int nc = m.nc, nf = m.nf;
+  T vc[5];
+  load5(V, nc, c, vc);
   const T vol = T(m.vol[c]);
-  int ent[kMaxSlots], nbr[kMaxSlots];  // slot entries; neighbour cell (-1: boundary face)
+  int ent[kMaxSlots];
+#pragma unroll
+  for (int s = 0; s < kMaxSlots; ++s) ent[s] = m.slot[(size_t)s * nc + c];
+
+  T g[5][3];
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) g[k][d] = T(0.0);
+  T mn[5], mx[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    mn[k] = T(1.0e300);
+    mx[k] = -T(1.0e300);
+  }
+
 #pragma unroll
   for (int s = 0; s < kMaxSlots; ++s) {
-    ent[s] = m.slot[(size_t)s * nc + c];
-    nbr[s] = -1;
-    const int f = ent[s] >> 1;
-    if (ent[s] >= 0 && f < m.nif) nbr[s] = (ent[s] & 1) ? m.fl[f] : m.fr[f];
-  }
-  double xcc[3];
-  if (kSecondOrder) load3(m.xc, nc, c, xcc);
-  // One conserved variable at a time: the five gradients, min/max and
-  // limiters are independent, and each accumulates over the slots in the
-  // same order as a joint loop would, so the results are the same bits with
-  // far fewer live registers (more waves in flight to hide gather latency).
-#pragma unroll 1
-  for (int k = 0; k < 5; ++k) {
-    const T* Vk = V + (size_t)k * nc;
-    const T vck = Vk[c];
-    T g[3] = {T(0.0), T(0.0), T(0.0)};
-    T mn = T(1.0e300), mx = -T(1.0e300);
+    const int e = ent[s];
+    if (e < 0) break;
+    const int f = e >> 1;
+    T n[3];
+    load3(m.fn, nf, f, n);
+    if (f < m.nif) {
+      const int side = e & 1;
+      const int nb = side ? m.fl[f] : m.fr[f];
+      T vn[5];
+      load5(V, nc, nb, vn);
+      const T* vl = side ? vn : vc;
+      const T* vr = side ? vc : vn;
 #pragma unroll
-    for (int s = 0; s < kMaxSlots; ++s) {
-      const int e = ent[s];
-      if (e < 0) break;
-      const int f = opaque(e >> 1);
-      T n[3];
-      load3(m.fn, nf, f, n);
-      if (nbr[s] >= 0) {
-        const int side = e & 1;
-        const T vn = Vk[nbr[s]];
-        const T vl = side ? vn : vck;
-        const T vr = side ? vck : vn;
+      for (int d = 0; d < 3; ++d)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const T gg = T(0.5) * (vl + vr) * n[d];
-          g[d] += side ? -gg / vol : gg / vol;
+        for (int k = 0; k < 5; ++k) {
+          const T gg = T(0.5) * (vl[k] + vr[k]) * n[d];
+          g[k][d] += side ? -gg / vol : gg / vol;
         }
-        if (kSecondOrder) {
-          const T fmin = vr < vl ? vr : vl;
-          const T fmax = vr > vl ? vr : vl;
-          mn = ma::smin(mn, fmin);
-          mx = ma::smax(mx, fmax);
-        }
-      } else {
+      if (kSecondOrder) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) g[d] += vck * n[d] / vol;
-        if (kSecondOrder) {
-          mn = ma::smin(mn, vck);
-          mx = ma::smax(mx, vck);
+        for (int k = 0; k < 5; ++k) {
+          const T fmin = vr[k] < vl[k] ? vr[k] : vl[k];
+          const T fmax = vr[k] > vl[k] ? vr[k] : vl[k];
+          mn[k] = ma::smin(mn[k], fmin);
+          mx[k] = ma::smax(mx[k], fmax);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 5; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) g[k][d] += vc[k] * n[d] / vol;
+      if (kSecondOrder) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          mn[k] = ma::smin(mn[k], vc[k]);
+          mx[k] = ma::smax(mx[k], vc[k]);
         }
       }
     }
-    // green_gauss_gradient_sum: gradient = 0; gradient += slot (GreenGauss.h:244-269)
+  }
+  // green_gauss_gradient_sum: gradient = 0; gradient += slot (GreenGauss.h:244-269)
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       T x = 0;
-      x += g[d];
-      g[d] = x;
+      x += g[k][d];
+      g[k][d] = x;
       grad[(size_t)(k * 3 + d) * nc + c] = x;
     }
-    if (!kSecondOrder) continue;
-    T lim = T(1.0);
+  if (!kSecondOrder) return;
+
+  double xcc[3];
+  load3(m.xc, nc, c, xcc);
+  T lim[5];
 #pragma unroll
-    for (int s = 0; s < kMaxSlots; ++s) {
-      const int e = ent[s];
-      if (e < 0) break;
-      const int f = opaque(e >> 1);
-      // displacement from absolute FP64 coordinates, then the arithmetic type
-      T disp[3], dist = 0;
+  for (int k = 0; k < 5; ++k) lim[k] = T(1.0);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        disp[d] = T(m.fx[(size_t)d * nf + f] - xcc[d]);
-        dist += disp[d] * disp[d];
-      }
+  for (int s = 0; s < kMaxSlots; ++s) {
+    const int e = ent[s];
+    if (e < 0) break;
+    const int f = e >> 1;
+    // displacement from absolute FP64 coordinates, then the arithmetic type
+    T disp[3], dist = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      disp[d] = T(m.fx[(size_t)d * nf + f] - xcc[d]);
+      dist += disp[d] * disp[d];
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
       T du = 0;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) du += disp[d] * g[d];
-      const T dumax = mx - vck;
-      const T dumin = mn - vck;
-      lim = ma::smin(lim, Limiter::limit(dumax, dumin, du, dist));
+      for (int d = 0; d < 3; ++d) du += disp[d] * g[k][d];
+      const T dumax = mx[k] - vc[k];
+      const T dumin = mn[k] - vc[k];
+      lim[k] = ma::smin(lim[k], Limiter::limit(dumax, dumin, du, dist));
     }
-    phi[(size_t)k * nc + c] = lim;
   }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) phi[(size_t)k * nc + c] = lim[k];
 }
 
-// amdgpu_waves_per_eu(6): register budget for six waves per SIMD (the FP64
-// second-order form then needs no scratch; eight would spill).
 template <bool kSecondOrder, class Limiter, class T>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_grad(DevMesh m, const T* __restrict__ V,
+__global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const T* __restrict__ V,
                                                  T* __restrict__ grad, T* __restrict__ phi,
                                                  int per_xcd) {
   // XCD-aware: each XCD takes a contiguous run of blocks, walked with stride
@@ -557,11 +568,8 @@ __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, in
   }
 }
 
-// Register budget: seven waves per SIMD in the FP32-arithmetic form (no
-// scratch); the FP64 form keeps the compiler's choice (its LDS buffer allows
-// four workgroups per CU in any case).
 template <bool kSecondOrder, class Visc, class T>
-__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 7 : 1))) void k_flux(DevMesh m, TileMap t, const T* __restrict__ V,
+__global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const T* __restrict__ V,
                                                 const T* __restrict__ grad,
                                                 const T* __restrict__ phi,
                                                 const T* __restrict__ bc,
