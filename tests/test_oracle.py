@@ -257,3 +257,34 @@ def test_uniform_flow_preserved_and_gradient_exact_for_linear_field():
     nx, ny, nz = kw["nx"], kw["ny"], kw["nz"]
     idx = np.arange(m.ncells).reshape(nx, ny, nz)[1:-1, 1:-1, 1:-1].ravel()
     assert np.allclose(g[idx, 0, :], [0.3, -0.2, 0.1], rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("viscous", [0, 1])
+def test_cfl_time_step_closed_form(viscous):
+    """CFL extension on a uniform state over a uniform box: every cell has
+    two faces of area dy*dz, dx*dz, dx*dy on each axis (boundary faces count),
+    so dt = cfl * vol / (sum_f(|u.n_f| + c|n_f|) + 4 kappa sum_f |n_f|^2 / vol)
+    with c = sqrt(gamma R T) and kappa = max(4/3, gamma/Pr) mu(T) / rho
+    (zero when inviscid) is the same for every cell."""
+    rho, u, v, w, T = 0.9, 120.0, -35.0, 20.0, 280.0
+    Rg, g = 287.05, 1.4
+    E = rho * (Rg / (g - 1.0)) * T + 0.5 * rho * (u * u + v * v + w * w)
+    nx, ny, nz, lx, ly, lz = 7, 5, 4, 0.7, 0.25, 0.4
+    kw = dict(problem_type=1, lx=lx, ly=ly, lz=lz, nx=nx, ny=ny, nz=nz, ntimesteps=1, dt=0.0,
+              second_order_space=0, viscous=viscous, bc_kind=(1, 1, 1, 1, 1, 1), use_inflow=1,
+              inflow=(rho, rho * u, rho * v, rho * w, E), prandtl=0.72, cfl=0.5)
+    s = O.OracleSolver(O.OracleMesh(O.make_options(**kw)))
+    s.init()
+    s.step(1)
+    dx, dy, dz = lx / nx, ly / ny, lz / nz
+    ax, ay, az = dy * dz, dx * dz, dx * dy
+    c = math.sqrt(g * Rg * T)
+    lc = 2 * (abs(u) * ax + abs(v) * ay + abs(w) * az) + 2 * c * (ax + ay + az)
+    lv = 2 * (ax * ax + ay * ay + az * az)
+    vol = dx * dy * dz
+    lam = lc
+    if viscous:
+        mu = 1.458e-6 * T * math.sqrt(T) / (T + 110.4)
+        lam += 4.0 * max(4.0 / 3.0, g / 0.72) * mu / rho * lv / vol
+    expect = 0.5 * vol / lam
+    assert s.time == pytest.approx(expect, rel=1e-12)
