@@ -717,11 +717,13 @@ __global__ __launch_bounds__(kBlock) void k_count_nonfinite(int n, const double*
 }
 
 // Plain streaming copy, the bandwidth ceiling the kernels above are compared
-// with (16-byte loads and stores, grid-stride).
+// with: one 16-byte load and store per thread over a grid covering the array
+// (on MI355X this form reaches ~6.3 TB/s read + write, a grid-stride loop
+// over 16 blocks per CU ~4.5 TB/s).
 __global__ __launch_bounds__(kBlock) void k_copy(long long n2, const double2* __restrict__ a,
                                                  double2* __restrict__ b) {
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) b[i] = a[i];
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n2) b[i] = a[i];
 }
 
 // ------------------------------------------------------------- host side --
@@ -2132,6 +2134,9 @@ extern "C" int ma_solver_profile(ma_solver* s, int nsteps, const char** names, i
 extern "C" ma_status ma_measure_copy(long long bytes, int reps, double* ms_per_copy) {
   if (bytes < 16 || reps < 1 || !ms_per_copy) return ma::fail(MA_ERR_INVALID, "ma_measure_copy: bad argument");
   const long long n2 = bytes / 16;
+  const long long blocks = (n2 + kBlock - 1) / kBlock;
+  if (blocks > 0x7fffffffll) return ma::fail(MA_ERR_INVALID, "ma_measure_copy: too many bytes");
+  const unsigned grid = (unsigned)blocks;
   double2 *a = nullptr, *b = nullptr;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2143,7 +2148,6 @@ extern "C" ma_status ma_measure_copy(long long bytes, int reps, double* ms_per_c
              hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
     rc = ma::fail(MA_ERR_DEVICE, "ma_measure_copy: stream/event creation failed");
   } else {
-    const int grid = 256 * 16;  // 16 blocks per CU
     (void)hipMemsetAsync(a, 0, n2 * 16, st);
     hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, st, n2, a, b);  // warm-up
     (void)hipEventRecord(e0, st);
