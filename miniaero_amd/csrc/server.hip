@@ -303,6 +303,11 @@ void scheduler(ma_server* S) {
       return;
     }
     bool progress = false;
+    // Cases stay in `act` until complete() hands them to the client (which
+    // may release, i.e. delete, them at once): nothing here touches a
+    // request after complete().
+    std::vector<Request*> keep;
+    keep.reserve(act.size());
     for (Request* r : act) {
       while (r->inflight > 0) {  // retire finished launches
         const hipError_t q = hipEventQuery(r->ev[r->head]);
@@ -338,11 +343,13 @@ void scheduler(ma_server* S) {
         progress = true;
       }
       if (r->inflight == 0 && (r->status != MA_OK || r->done >= r->total)) {
-        complete(S, r);
+        complete(S, r);  // last use of r
         progress = true;
+      } else {
+        keep.push_back(r);
       }
     }
-    act.erase(std::remove_if(act.begin(), act.end(), [](Request* r) { return r->finished; }), act.end());
+    act.swap(keep);
     if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(50));
   }
 }
