@@ -293,7 +293,8 @@ def bench_served(args, M, stream, copy_gbs) -> dict:
     come.  value = sum over cases of cells * 4 * steps / (last completion -
     first submission); latency = completion - submission."""
     import numpy as np
-    srv = M.Server(hbm_budget_bytes=args.hbm_budget, max_active=args.max_active)
+    srv = M.Server(hbm_budget_bytes=args.hbm_budget, max_active=args.max_active,
+                   setup_threads=args.setup_threads)
     # one small case first, so that module loading is not in the first request's latency
     warm = stream[0][1].replace(nx=8, ny=4, nz=2, ntimesteps=2)
     wid = srv.submit(warm)
@@ -371,7 +372,8 @@ def bench_served(args, M, stream, copy_gbs) -> dict:
                   {"min": int(cells.min()), "mean": float(cells.mean()), "max": int(cells.max())},
               "physics": "Roe, Green-Gauss, Venkat, 2nd order; half viscous flat plate, half slip wall",
               "precision": "fp32 flux/gradient/limiter, fp64 state",
-              "max_active": args.max_active or 16, "region_bytes": st0["region_size"],
+              "max_active": args.max_active or 16, "setup_threads": args.setup_threads,
+              "region_bytes": st0["region_size"],
               "peak_region_bytes_used": int(peak_used),
               "case_latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                                  "max": float(lat.max())},
@@ -402,6 +404,8 @@ def main() -> int:
     ap.add_argument("--cases", type=int, default=200, help="served workload: requests in the stream")
     ap.add_argument("--max-active", type=int, default=0, help="served workload: cases on the GPU at once")
     ap.add_argument("--hbm-budget", type=float, default=200e9, help="served workload: device bytes for cases")
+    ap.add_argument("--setup-threads", type=int, default=12,
+                    help="served workload: host threads generating and planning requests")
     ap.add_argument("--wg-per-cu", type=int, default=None,
                     help="resident workgroups per CU for the gradient/flux kernels (0: one per tile)")
     ap.add_argument("--copy-ceiling", action="store_true",

@@ -218,8 +218,10 @@ ma_status ma_batch_get_case_time(ma_solver* s, int k, double* time, double* last
 /* ---- served form: a request loop over independent cases ---- */
 
 /* A server keeps admitted cases' state on the device between steps and runs
- * them concurrently, each on its own stream, while new requests arrive
- * (continuous admission).  Requests are planned on host threads (mesh
+ * them together while new requests arrive (continuous batching): the cases
+ * of one scheme (order, viscous, limiter, precision, Prandtl) are advanced
+ * by the same launches, one RK4 step at a time; a case joins at the next step
+ * after its admission and leaves after its last one.  Requests are planned on host threads (mesh
  * generated from the options as Parallel3DMesh does, layout, exact device
  * size) and admitted in submission order when a slot is free and their
  * device bytes fit in the server's region of HBM, which is allocated once at
@@ -233,7 +235,7 @@ typedef struct {
   long long hbm_budget_bytes; /* device region for admitted cases; 0 = 200e9, capped by free HBM */
   int max_active;             /* cases on the device at once; 0 = 16 */
   int setup_threads;          /* host threads planning requests; 0 = 4 */
-  int steps_per_launch;       /* RK4 steps per captured graph launch (even); 0 = 10 */
+  int steps_in_flight;        /* RK4 steps queued ahead per scheme group; 0 = 2 */
 } ma_server_config;
 
 typedef struct {

@@ -40,7 +40,7 @@ EXPORTS = (
 
 class ma_server_config(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_longlong), ("max_active", ctypes.c_int),
-                ("setup_threads", ctypes.c_int), ("steps_per_launch", ctypes.c_int)]
+                ("setup_threads", ctypes.c_int), ("steps_in_flight", ctypes.c_int)]
 
 
 class ma_case_result(ctypes.Structure):

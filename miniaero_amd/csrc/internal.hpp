@@ -20,5 +20,22 @@ ma_status create_from_plan(PlannedCase* pc, void* region, size_t region_bytes, m
 // ma_solver_step without the argument checks.
 ma_status solver_advance(ma_solver* s, int nsteps);
 hipStream_t solver_stream(const ma_solver* s);
+// Which of the case's two state buffers holds sol_n (0 or 1).
+int solver_cur(const ma_solver* s);
+// Records that `steps` steps ran outside the solver's own launches (in a
+// Multi): flips the state buffer for an odd count, adds to the step count.
+void solver_set_progress(ma_solver* s, int steps);
+
+// Several single-case solvers of one scheme (second order, viscous, limiter,
+// precision, Prandtl) advanced by the same launches on `stream`.
+struct Multi;
+ma_status multi_create(const ma_options& scheme, int capacity, hipStream_t stream, Multi** out);
+// Sets the cases of the following steps (uploaded on the stream, after the
+// steps already queued).  Case k's state buffer for sol_n at a step of
+// parity p is parity[k] ^ p.
+ma_status multi_set(Multi* M, ma_solver* const* cases, const int* parity, int n);
+// One RK4 step of every case (asynchronous on the stream).
+ma_status multi_step(Multi* M, int parity);
+void multi_destroy(Multi* M);
 
 }  // namespace ma

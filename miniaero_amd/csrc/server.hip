@@ -8,17 +8,19 @@
  *   (ma_mesh_generate), plan the device layout and compute the exact device
  *   bytes (ma::plan_case) -> admission in submission order, when a slot is
  *   free and the bytes fit in the server's region -> build inside the region
- *   on the case's own stream, initial state, graph capture -> scheduler
- *   thread: keeps at most two launches of steps_per_launch steps in flight
- *   per case, so every admitted case has queued work and cases overlap on
- *   the device -> when a case has run its steps, its state is copied to the
- *   host, its region block is returned and the next request can be admitted.
+ *   on the case's own stream and set its initial state -> scheduler thread:
+ *   the cases of one scheme form a group whose single set of launches
+ *   advances all of them by one RK4 step (continuous batching; each
+ *   workgroup finds its case in a device table); a case joins its group at
+ *   the next step and leaves after its last one, its state is then copied to
+ *   the host, its region block returned and the next request admitted.
  *
  * The region is one hipMalloc made at creation, so admitting and retiring
  * cases never allocates or frees device memory (hipFree can wait for the
  * whole device, which would stall the other cases).  Each case owns its
- * buffers and stream and runs the same kernels in the same order as a
- * ma_solver_create run of it alone, so its result is bit for bit the same.
+ * buffers and goes through the same per-cell and per-face arithmetic in the
+ * same order as a ma_solver_create run of it alone, so its result is bit for
+ * bit the same.
  */
 #include <hip/hip_runtime.h>
 
@@ -103,11 +105,34 @@ struct Request {
   size_t bytes = 0, offset = 0;
   bool has_region = false, finished = false;
   ma_solver* solver = nullptr;
-  int total = 0, enqueued = 0, done = 0;
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  int ev_steps[2] = {0, 0};
-  int head = 0, inflight = 0;
+  int total = 0, done = 0;  // steps asked for, steps run
+  int remaining = 0;        // steps not yet queued
+  int parity = 0;           // state buffer of sol_n = parity ^ the group's step parity
   std::vector<double> U;
+};
+
+// Cases that can share launches: same order, viscous flag, limiter,
+// arithmetic and Prandtl number.
+bool same_scheme(const ma_options& a, const ma_options& b) {
+  const double pa = a.prandtl > 0.0 ? a.prandtl : 0.71, pb = b.prandtl > 0.0 ? b.prandtl : 0.71;
+  return a.second_order_space == b.second_order_space && a.viscous == b.viscous && a.limiter == b.limiter &&
+         a.precision == b.precision && pa == pb;
+}
+
+// The cases of one scheme and the launches that advance them together.
+struct Group {
+  ma_options scheme{};
+  hipStream_t stream = nullptr;
+  ma::Multi* multi = nullptr;
+  std::vector<Request*> act;  // in the launches of the next step
+  bool dirty = true;          // act changed since the last upload
+  int parity = 0;             // parity of the next step
+  struct Step {
+    hipEvent_t ev;
+    std::vector<Request*> finishing;  // cases whose last step this is
+  };
+  std::deque<Step> inflight;
+  std::vector<hipEvent_t> spare;
 };
 
 }  // namespace
@@ -162,11 +187,6 @@ void fail_request(ma_server* S, Request* r, ma_status st, const std::string& err
 void retire(ma_server* S, Request* r) {
   if (r->solver) ma_solver_destroy(r->solver);
   r->solver = nullptr;
-  for (auto& e : r->ev)
-    if (e) {
-      (void)hipEventDestroy(e);
-      e = nullptr;
-    }
   std::lock_guard<std::mutex> lk(S->mu);
   if (r->has_region) {
     S->region.release(r->offset, r->bytes);
@@ -241,10 +261,6 @@ void setup_worker(ma_server* S) {
     ma::destroy_plan(pc);
     r->solver = s;
     if (!st) st = ma_solver_initialize(s);
-    if (!st) st = ma_solver_set_graph(s, S->cfg.steps_per_launch);
-    for (auto& e : r->ev)
-      if (!st && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
-        st = ma::fail(MA_ERR_DEVICE, "hipEventCreate failed");
     if (st) {
       err = ma::last_error();
       retire(S, r);
@@ -276,80 +292,168 @@ void complete(ma_server* S, Request* r) {
   mark_finished(S, r);
 }
 
+void fail_group(ma_server* S, Group& g, const std::string& why) {
+  (void)hipStreamSynchronize(g.stream);
+  std::vector<Request*> all(g.act);
+  for (auto& st : g.inflight) {
+    all.insert(all.end(), st.finishing.begin(), st.finishing.end());
+    g.spare.push_back(st.ev);
+  }
+  g.inflight.clear();
+  g.act.clear();
+  g.dirty = true;
+  for (Request* r : all) {
+    r->status = MA_ERR_DEVICE;
+    r->error = why;
+    complete(S, r);
+  }
+}
+
+// The request loop's device side: every group queues one RK step of all its
+// cases at a time (at most steps_in_flight ahead), cases join a group at its
+// next step and leave after their last one.
 void scheduler(ma_server* S) {
   (void)hipSetDevice(S->device);
-  std::vector<Request*> act;
+  std::vector<std::unique_ptr<Group>> groups;
+  auto idle = [&] {
+    for (auto& g : groups)
+      if (!g->act.empty() || !g->inflight.empty()) return false;
+    return true;
+  };
   for (;;) {
+    std::vector<Request*> joining;
     bool stopping = false;
     {
       std::unique_lock<std::mutex> lk(S->mu);
-      if (act.empty()) S->cv_sched.wait(lk, [&] { return S->stop || !S->incoming.empty(); });
-      while (!S->incoming.empty()) {
-        act.push_back(S->incoming.front());
-        S->incoming.pop_front();
-      }
+      if (idle()) S->cv_sched.wait(lk, [&] { return S->stop || !S->incoming.empty(); });
+      joining.assign(S->incoming.begin(), S->incoming.end());
+      S->incoming.clear();
       stopping = S->stop;
     }
     if (stopping) {
-      // launched work runs out; the cases are dropped
-      for (Request* r : act) {
-        (void)hipStreamSynchronize(ma::solver_stream(r->solver));
+      for (auto& g : groups) {
+        (void)hipStreamSynchronize(g->stream);
+        std::vector<Request*> all(g->act);
+        for (auto& st : g->inflight) all.insert(all.end(), st.finishing.begin(), st.finishing.end());
+        all.insert(all.end(), joining.begin(), joining.end());
+        joining.clear();
+        for (Request* r : all) {
+          r->status = MA_ERR_INVALID;
+          r->error = "server stopped before the case finished";
+          retire(S, r);
+          std::lock_guard<std::mutex> lk(S->mu);
+          mark_finished(S, r);
+        }
+      }
+      for (Request* r : joining) {
         r->status = MA_ERR_INVALID;
         r->error = "server stopped before the case finished";
         retire(S, r);
         std::lock_guard<std::mutex> lk(S->mu);
         mark_finished(S, r);
       }
+      for (auto& g : groups) {
+        for (auto& st : g->inflight) (void)hipEventDestroy(st.ev);
+        for (hipEvent_t e : g->spare) (void)hipEventDestroy(e);
+        ma::multi_destroy(g->multi);
+        (void)hipStreamDestroy(g->stream);
+      }
       return;
     }
     bool progress = false;
-    // Cases stay in `act` until complete() hands them to the client (which
-    // may release, i.e. delete, them at once): nothing here touches a
-    // request after complete().
-    std::vector<Request*> keep;
-    keep.reserve(act.size());
-    for (Request* r : act) {
-      while (r->inflight > 0) {  // retire finished launches
-        const hipError_t q = hipEventQuery(r->ev[r->head]);
-        if (q == hipErrorNotReady) break;
-        if (q != hipSuccess) {
-          r->status = MA_ERR_DEVICE;
-          r->error = std::string("case stream: ") + hipGetErrorString(q);
-          (void)hipStreamSynchronize(ma::solver_stream(r->solver));
-          r->inflight = 0;
-          break;
-        }
-        r->done += r->ev_steps[r->head];
-        r->head ^= 1;
-        --r->inflight;
-        progress = true;
+    for (Request* r : joining) {
+      progress = true;
+      if (r->total == 0) {
+        complete(S, r);  // the initial state is the result
+        continue;
       }
-      while (r->status == MA_OK && r->inflight < 2 && r->enqueued < r->total) {  // keep it fed
-        const int n = std::min(S->cfg.steps_per_launch, r->total - r->enqueued);
-        const int slot = (r->head + r->inflight) & 1;
-        ma_status st = ma::solver_advance(r->solver, n);
-        if (!st && hipEventRecord(r->ev[slot], ma::solver_stream(r->solver)) != hipSuccess)
-          st = ma::fail(MA_ERR_DEVICE, "hipEventRecord failed");
+      Group* g = nullptr;
+      for (auto& x : groups)
+        if (same_scheme(x->scheme, r->opt)) g = x.get();
+      if (!g) {
+        std::unique_ptr<Group> ng(new Group());
+        ng->scheme = r->opt;
+        ma_status st = MA_OK;
+        if (hipStreamCreateWithFlags(&ng->stream, hipStreamNonBlocking) != hipSuccess)
+          st = ma::fail(MA_ERR_DEVICE, "hipStreamCreate failed");
+        if (!st) st = ma::multi_create(r->opt, S->cfg.max_active, ng->stream, &ng->multi);
         if (st) {
+          if (ng->stream) (void)hipStreamDestroy(ng->stream);
           r->status = st;
           r->error = ma::last_error();
-          (void)hipStreamSynchronize(ma::solver_stream(r->solver));
-          r->inflight = 0;
+          complete(S, r);
+          continue;
+        }
+        groups.push_back(std::move(ng));
+        g = groups.back().get();
+      }
+      r->parity = ma::solver_cur(r->solver) ^ g->parity;
+      r->remaining = r->total;
+      g->act.push_back(r);
+      g->dirty = true;
+    }
+    for (auto& gp : groups) {
+      Group& g = *gp;
+      while (!g.inflight.empty()) {  // finished steps
+        const hipError_t q = hipEventQuery(g.inflight.front().ev);
+        if (q == hipErrorNotReady) break;
+        if (q != hipSuccess) {
+          fail_group(S, g, std::string("group stream: ") + hipGetErrorString(q));
           break;
         }
-        r->ev_steps[slot] = n;
-        ++r->inflight;
-        r->enqueued += n;
+        Group::Step st = g.inflight.front();
+        g.inflight.pop_front();
+        g.spare.push_back(st.ev);
+        for (Request* r : st.finishing) {
+          ma::solver_set_progress(r->solver, r->total);
+          r->done = r->total;
+          complete(S, r);  // last use of r
+        }
         progress = true;
       }
-      if (r->inflight == 0 && (r->status != MA_OK || r->done >= r->total)) {
-        complete(S, r);  // last use of r
+      while (!g.act.empty() && (int)g.inflight.size() < S->cfg.steps_in_flight) {  // queue steps
+        ma_status st = MA_OK;
+        if (g.dirty) {
+          std::vector<ma_solver*> sv;
+          std::vector<int> par;
+          for (Request* r : g.act) {
+            sv.push_back(r->solver);
+            par.push_back(r->parity);
+          }
+          st = ma::multi_set(g.multi, sv.data(), par.data(), (int)sv.size());
+          g.dirty = false;
+        }
+        if (!st) st = ma::multi_step(g.multi, g.parity);
+        hipEvent_t ev = nullptr;
+        if (!st) {
+          if (!g.spare.empty()) {
+            ev = g.spare.back();
+            g.spare.pop_back();
+          } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            st = ma::fail(MA_ERR_DEVICE, "hipEventCreate failed");
+          }
+        }
+        if (!st && hipEventRecord(ev, g.stream) != hipSuccess) st = ma::fail(MA_ERR_DEVICE, "hipEventRecord failed");
+        if (st) {
+          if (ev) g.spare.push_back(ev);
+          fail_group(S, g, ma::last_error());
+          break;
+        }
+        Group::Step step{ev, {}};
+        std::vector<Request*> keep;
+        for (Request* r : g.act) {
+          if (--r->remaining == 0) step.finishing.push_back(r);
+          else keep.push_back(r);
+        }
+        if (keep.size() != g.act.size()) {
+          g.act.swap(keep);
+          g.dirty = true;
+        }
+        g.inflight.push_back(std::move(step));
+        g.parity ^= 1;
         progress = true;
-      } else {
-        keep.push_back(r);
       }
     }
-    act.swap(keep);
     if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(50));
   }
 }
@@ -360,13 +464,12 @@ extern "C" ma_status ma_server_create(const ma_server_config* cfg, ma_server** o
   if (!out) return ma::fail(MA_ERR_INVALID, "ma_server_create: null argument");
   *out = nullptr;
   ma_server_config c = cfg ? *cfg : ma_server_config{};
-  if (c.hbm_budget_bytes < 0 || c.max_active < 0 || c.setup_threads < 0 || c.steps_per_launch < 0 ||
-      (c.steps_per_launch & 1))
-    return ma::fail(MA_ERR_INVALID, "ma_server_create: negative setting or odd steps_per_launch");
+  if (c.hbm_budget_bytes < 0 || c.max_active < 0 || c.setup_threads < 0 || c.steps_in_flight < 0)
+    return ma::fail(MA_ERR_INVALID, "ma_server_create: negative setting");
   if (c.hbm_budget_bytes == 0) c.hbm_budget_bytes = 200000000000ll;
   if (c.max_active == 0) c.max_active = 16;
   if (c.setup_threads == 0) c.setup_threads = 4;
-  if (c.steps_per_launch == 0) c.steps_per_launch = 10;
+  if (c.steps_in_flight == 0) c.steps_in_flight = 2;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return ma::fail(MA_ERR_DEVICE, "no HIP device available");

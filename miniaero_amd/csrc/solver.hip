@@ -55,6 +55,7 @@
 
 #include "../../include/miniaero.h"
 #include "common.hpp"
+#include "internal.hpp"
 #include "physics.hpp"
 
 namespace {
@@ -225,17 +226,22 @@ __global__ __launch_bounds__(kBlock) void k_init(DevMesh m, const IcParams* __re
 // (TimeSolverExplicitRK4.h:106-128, 355-357) fused with ComputePrimitives.
 // The stage state is FP64; V is stored in the arithmetic type T.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_prim(int nc, const double* __restrict__ Un,
-                                                 const double* __restrict__ R, double alpha,
-                                                 T* __restrict__ V) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
+__device__ __forceinline__ void prim_cell(int nc, const double* __restrict__ Un, const double* __restrict__ R,
+                                          double alpha, T* __restrict__ V, int c) {
   double u[5], v[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) u[k] = Un[(size_t)k * nc + c] + alpha * R[(size_t)k * nc + c];
   ma::primitives(u, v);
 #pragma unroll
   for (int k = 0; k < 5; ++k) V[(size_t)k * nc + c] = T(v[k]);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_prim(int nc, const double* __restrict__ Un,
+                                                 const double* __restrict__ R, double alpha,
+                                                 T* __restrict__ V) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < nc) prim_cell(nc, Un, R, alpha, V, c);
 }
 
 // Green-Gauss gradient (GreenGauss.h:51-270), then, for second order, the
@@ -429,12 +435,10 @@ __device__ __forceinline__ void interior_flux(const DevMesh& m, const T* __restr
 // newtonian viscous part is applied even in inviscid runs,
 // TimeSolverExplicitRK4.h:457-460).  Written to bc[5][nbf].
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const T* __restrict__ V, double prandtl,
-                                               const Inflow* __restrict__ inflows,
-                                               const int* __restrict__ bc_case, T* __restrict__ bc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void bc_face(const DevMesh& m, const T* __restrict__ V, double prandtl,
+                                        const Inflow* __restrict__ inflows, const int* __restrict__ bc_case,
+                                        T* __restrict__ bc, int i) {
   const int nbf = m.nf - m.nif;
-  if (i >= nbf) return;
   const int f = m.nif + i;
   const int c = m.fl[f];
   const int kind = m.fkind[f];
@@ -490,6 +494,14 @@ __global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const T* __restrict__ 
   }
 #pragma unroll
   for (int k = 0; k < 5; ++k) bc[(size_t)k * nbf + i] = out[k];
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const T* __restrict__ V, double prandtl,
+                                               const Inflow* __restrict__ inflows,
+                                               const int* __restrict__ bc_case, T* __restrict__ bc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m.nf - m.nif) bc_face(m, V, prandtl, inflows, bc_case, bc, i);
 }
 
 // Residual and RK accumulate of one tile of at most kTile cells:
@@ -599,11 +611,11 @@ __global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const T* _
 //   dt_c = vol / (lc + 4 kappa lv / vol),  kappa = max(4/3, gamma/Pr) mu / rho
 // (kappa = 0 for inviscid runs: convective plus viscous spectral radius),
 // reduced to the minimum per block.
-__global__ __launch_bounds__(kTile) void k_dt_partial(DevMesh m, TileMap t,
-                                                      const double* __restrict__ Un, int viscous,
-                                                      double prandtl, double* __restrict__ partial) {
-  __shared__ double red[kTile / 64];
-  const int tile = blockIdx.x;  // one workgroup per tile: a tile never spans two cases
+// One workgroup of kTile threads per tile (a tile never spans two cases);
+// red: kTile / 64 doubles of shared memory.
+__device__ __forceinline__ void dt_tile(const DevMesh& m, const TileMap& t, int tile,
+                                        const double* __restrict__ Un, int viscous, double prandtl,
+                                        double* __restrict__ partial, double* red) {
   const int c = t.cptr[tile] + threadIdx.x;
   double dtc = 1e300;
   if (c < t.cptr[tile + 1]) {
@@ -645,16 +657,21 @@ __global__ __launch_bounds__(kTile) void k_dt_partial(DevMesh m, TileMap t,
   }
 }
 
+__global__ __launch_bounds__(kTile) void k_dt_partial(DevMesh m, TileMap t,
+                                                      const double* __restrict__ Un, int viscous,
+                                                      double prandtl, double* __restrict__ partial) {
+  __shared__ double red[kTile / 64];
+  dt_tile(m, t, blockIdx.x, Un, viscous, prandtl, partial, red);
+}
+
 // One workgroup per case: dt = cfl * (minimum over the case's tiles) when the
 // case runs at a CFL number (cfl > 0), else its fixed dt; then time += dt
 // (TimeSolverExplicitRK4.h:343).
-__global__ __launch_bounds__(kBlock) void k_dt_final(const double* __restrict__ partial,
-                                                     const int* __restrict__ case_tiles,
-                                                     const double* __restrict__ cfl,
-                                                     double* __restrict__ dt_dev,
-                                                     double* __restrict__ time_dev) {
-  __shared__ double red[kBlock / 64];
-  const int k = blockIdx.x;
+// One workgroup of kBlock threads per case; red: kBlock / 64 doubles of
+// shared memory.
+__device__ __forceinline__ void dt_case(const double* __restrict__ partial, const int* __restrict__ case_tiles,
+                                        const double* __restrict__ cfl, double* __restrict__ dt_dev,
+                                        double* __restrict__ time_dev, int k, double* red) {
   const double cf = cfl[k];
   double x = 1e300;
   if (cf > 0.0)
@@ -671,6 +688,15 @@ __global__ __launch_bounds__(kBlock) void k_dt_final(const double* __restrict__ 
     dt_dev[k] = dt;
     time_dev[k] = time_dev[k] + dt;
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dt_final(const double* __restrict__ partial,
+                                                     const int* __restrict__ case_tiles,
+                                                     const double* __restrict__ cfl,
+                                                     double* __restrict__ dt_dev,
+                                                     double* __restrict__ time_dev) {
+  __shared__ double red[kBlock / 64];
+  dt_case(partial, case_tiles, cfl, dt_dev, time_dev, blockIdx.x, red);
 }
 
 // Residual norms: per tile, the five sums of R_k^2 over its cells (wavefront
@@ -724,6 +750,103 @@ __global__ __launch_bounds__(kBlock) void k_copy(long long n2, const double2* __
                                                  double2* __restrict__ b) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n2) b[i] = a[i];
+}
+
+// ------------------------------------------------- multi-case launches --
+// Several independent single-case runs advanced by the same launches (the
+// served form's continuous batching): each workgroup finds its case from a
+// prefix table and runs the single-case body on that case's own arrays.
+
+struct CaseView {
+  DevMesh m;
+  TileMap t;
+  double* U[2];
+  double* R;
+  void *V, *grad, *phi, *bc;  // arithmetic type of the launch
+  double *dt, *time, *partial;
+  const double* cfl;
+  const int* case_tiles;  // {0, ntiles}
+  const Inflow* inflow;
+  const int* bc_case;
+  int parity;  // U[parity ^ step parity] holds the case's sol_n
+};
+
+// Prefix table: start[k] = first global block of case k, start[n] = total.
+__device__ __forceinline__ int find_case(const int* __restrict__ start, int n, int g) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_prim_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
+                                                       int n, double alpha, int parity) {
+  const int g = blockIdx.x;
+  const int k = find_case(start, n, g);
+  const CaseView& v = cv[k];
+  const int c = (g - start[k]) * kBlock + threadIdx.x;
+  if (c < v.m.nc) prim_cell(v.m.nc, v.U[v.parity ^ parity], v.R, alpha, static_cast<T*>(v.V), c);
+}
+
+template <bool kSecondOrder, class Limiter, class T>
+__global__ __launch_bounds__(kBlock) void k_grad_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
+                                                       int n, int per_xcd) {
+  const int g = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  if (g >= start[n]) return;
+  const int k = find_case(start, n, g);
+  const CaseView& v = cv[k];
+  const int c = (g - start[k]) * kBlock + threadIdx.x;
+  if (c < v.m.nc)
+    grad_cell<kSecondOrder, Limiter, T>(v.m, static_cast<const T*>(v.V), static_cast<T*>(v.grad),
+                                        static_cast<T*>(v.phi), c);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_bc_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
+                                                     int n, double prandtl) {
+  const int g = blockIdx.x;
+  const int k = find_case(start, n, g);
+  const CaseView& v = cv[k];
+  const int i = (g - start[k]) * kBlock + threadIdx.x;
+  if (i < v.m.nf - v.m.nif)
+    bc_face(v.m, static_cast<const T*>(v.V), prandtl, v.inflow, v.bc_case, static_cast<T*>(v.bc), i);
+}
+
+template <bool kSecondOrder, class Visc, class T>
+__global__ __launch_bounds__(kTile) void k_flux_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
+                                                      int n, int per_xcd, int parity, int first_stage, double beta,
+                                                      Visc visc) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int g = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  if (g >= start[n]) return;  // whole workgroup, before any barrier
+  const int k = find_case(start, n, g);
+  const CaseView& v = cv[k];
+  const int cur = v.parity ^ parity;
+  double* Uacc = v.U[1 - cur];
+  flux_tile<kSecondOrder, Visc, T>(v.m, v.t, g - start[k], reinterpret_cast<T*>(lds_raw),
+                                   static_cast<const T*>(v.V), static_cast<const T*>(v.grad),
+                                   static_cast<const T*>(v.phi), static_cast<const T*>(v.bc),
+                                   first_stage ? v.U[cur] : Uacc, Uacc, v.R, v.dt, beta, visc);
+}
+
+__global__ __launch_bounds__(kTile) void k_dt_partial_multi(const CaseView* __restrict__ cv,
+                                                            const int* __restrict__ start, int n, int parity,
+                                                            int viscous, double prandtl) {
+  __shared__ double red[kTile / 64];
+  const int g = blockIdx.x;
+  const int k = find_case(start, n, g);
+  const CaseView& v = cv[k];
+  dt_tile(v.m, v.t, g - start[k], v.U[v.parity ^ parity], viscous, prandtl, v.partial, red);
+}
+
+__global__ __launch_bounds__(kBlock) void k_dt_final_multi(const CaseView* __restrict__ cv) {
+  __shared__ double red[kBlock / 64];
+  const CaseView& v = cv[blockIdx.x];
+  dt_case(v.partial, v.case_tiles, v.cfl, v.dt, v.time, 0, red);
 }
 
 // ------------------------------------------------------------- host side --
@@ -2264,5 +2387,187 @@ ma_status solver_advance(ma_solver* s, int nsteps) {
 }
 
 hipStream_t solver_stream(const ma_solver* s) { return s->stream; }
+
+}  // namespace ma
+
+// Multi-case launches (continuous batching of the served form).
+namespace ma {
+
+struct Multi {
+  ma_options scheme{};
+  bool fp32 = false, any_cfl = false;
+  int capacity = 0, n = 0;
+  hipStream_t stream = nullptr;
+  CaseView* d_views = nullptr;  // capacity
+  int* d_start = nullptr;       // 3 x (capacity + 1): cell blocks, boundary-face blocks, tiles
+  // pinned staging, two sets used in turn so that a new table can be staged
+  // while the previous upload is still queued behind running steps
+  CaseView* h_views[2] = {nullptr, nullptr};
+  int* h_start[2] = {nullptr, nullptr};
+  hipEvent_t uploaded[2] = {nullptr, nullptr};
+  int slot = 0;
+  int totals[3] = {0, 0, 0};
+  int max_faces = 1;
+};
+
+ma_status multi_create(const ma_options& scheme, int capacity, hipStream_t stream, Multi** out) {
+  *out = nullptr;
+  Multi* M = new (std::nothrow) Multi();
+  if (!M) return ma::fail(MA_ERR_NOMEM, "multi: out of host memory");
+  M->scheme = scheme;
+  M->fp32 = scheme.precision == MA_PRECISION_FP32;
+  M->capacity = capacity;
+  M->stream = stream;
+  const size_t nstart = (size_t)3 * (capacity + 1);
+  bool ok = hipMalloc((void**)&M->d_views, capacity * sizeof(CaseView)) == hipSuccess &&
+            hipMalloc((void**)&M->d_start, nstart * sizeof(int)) == hipSuccess;
+  for (int b = 0; b < 2 && ok; ++b)
+    ok = hipHostMalloc((void**)&M->h_views[b], capacity * sizeof(CaseView), hipHostMallocDefault) == hipSuccess &&
+         hipHostMalloc((void**)&M->h_start[b], nstart * sizeof(int), hipHostMallocDefault) == hipSuccess &&
+         hipEventCreateWithFlags(&M->uploaded[b], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    multi_destroy(M);
+    return ma::fail(MA_ERR_NOMEM, "multi: device or pinned allocation failed");
+  }
+  *out = M;
+  return MA_OK;
+}
+
+void multi_destroy(Multi* M) {
+  if (!M) return;
+  for (int b = 0; b < 2; ++b) {
+    if (M->uploaded[b]) {
+      (void)hipEventSynchronize(M->uploaded[b]);
+      (void)hipEventDestroy(M->uploaded[b]);
+    }
+    if (M->h_views[b]) (void)hipHostFree(M->h_views[b]);
+    if (M->h_start[b]) (void)hipHostFree(M->h_start[b]);
+  }
+  if (M->d_views) (void)hipFree(M->d_views);
+  if (M->d_start) (void)hipFree(M->d_start);
+  delete M;
+}
+
+ma_status multi_set(Multi* M, ma_solver* const* cases, const int* parity, int n) {
+  if (n < 0 || n > M->capacity) return ma::fail(MA_ERR_INVALID, "multi: too many cases");
+  const int b = M->slot;
+  MA_HIP(hipEventSynchronize(M->uploaded[b]));  // this staging set's last upload has run
+  int* st = M->h_start[b];
+  const int cap1 = M->capacity + 1;
+  st[0] = st[cap1] = st[2 * cap1] = 0;
+  M->any_cfl = false;
+  M->max_faces = 1;
+  for (int k = 0; k < n; ++k) {
+    ma_solver* s = cases[k];
+    if (s->ncases != 1 || s->fp32 != M->fp32) return ma::fail(MA_ERR_INVALID, "multi: incompatible case");
+    CaseView& v = M->h_views[b][k];
+    v.m = s->m;
+    v.t = s->t;
+    v.U[0] = s->U[0];
+    v.U[1] = s->U[1];
+    v.R = s->R;
+    v.V = s->V;
+    v.grad = s->grad;
+    v.phi = s->phi;
+    v.bc = s->bc;
+    v.dt = s->dt_dev;
+    v.time = s->time_dev;
+    v.partial = s->partial;
+    v.cfl = s->cfl_dev;
+    v.case_tiles = s->case_tiles;
+    v.inflow = s->inflow_dev;
+    v.bc_case = s->bc_case;
+    v.parity = parity[k];
+    st[k + 1] = st[k] + nblocks(s->m.nc);
+    st[cap1 + k + 1] = st[cap1 + k] + nblocks(s->m.nf - s->m.nif);
+    st[2 * cap1 + k + 1] = st[2 * cap1 + k] + s->t.ntiles;
+    M->any_cfl = M->any_cfl || s->any_cfl;
+    M->max_faces = std::max(M->max_faces, s->t.max_faces);
+  }
+  M->n = n;
+  M->totals[0] = st[n];
+  M->totals[1] = st[cap1 + n];
+  M->totals[2] = st[2 * cap1 + n];
+  if (n > 0) {
+    MA_HIP(hipMemcpyAsync(M->d_views, M->h_views[b], n * sizeof(CaseView), hipMemcpyHostToDevice, M->stream));
+    MA_HIP(hipMemcpyAsync(M->d_start, st, (size_t)3 * cap1 * sizeof(int), hipMemcpyHostToDevice, M->stream));
+  }
+  MA_HIP(hipEventRecord(M->uploaded[b], M->stream));
+  M->slot = 1 - b;
+  return MA_OK;
+}
+
+}  // namespace ma
+
+namespace {
+
+template <bool SO, class Visc, class Lim, class T>
+void multi_step_t(ma::Multi* M, int parity) {
+  constexpr double alpha[4] = {0.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};
+  constexpr double beta[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+  const int n = M->n, cap1 = M->capacity + 1;
+  const int* sc = M->d_start;
+  const int* sb = M->d_start + cap1;
+  const int* stl = M->d_start + 2 * cap1;
+  const double prandtl = effective_prandtl(M->scheme);
+  if (M->any_cfl)
+    hipLaunchKernelGGL(k_dt_partial_multi, dim3(M->totals[2]), dim3(kTile), 0, M->stream, M->d_views, stl, n,
+                       parity, M->scheme.viscous, prandtl);
+  hipLaunchKernelGGL(k_dt_final_multi, dim3(n), dim3(kBlock), 0, M->stream, M->d_views);
+  Visc visc{prandtl};
+  const int per_cells = (M->totals[0] + kXcds - 1) / kXcds;
+  const int per_tiles = (M->totals[2] + kXcds - 1) / kXcds;
+  const size_t lds = (size_t)M->max_faces * 5 * sizeof(T);
+  for (int irk = 0; irk < 4; ++irk) {
+    hipLaunchKernelGGL(k_prim_multi<T>, dim3(M->totals[0]), dim3(kBlock), 0, M->stream, M->d_views, sc, n,
+                       alpha[irk], parity);
+    if (SO || Visc::isViscous)
+      hipLaunchKernelGGL((k_grad_multi<SO, Lim, T>), dim3(per_cells * kXcds), dim3(kBlock), 0, M->stream,
+                         M->d_views, sc, n, per_cells);
+    if (M->totals[1] > 0)
+      hipLaunchKernelGGL(k_bc_multi<T>, dim3(M->totals[1]), dim3(kBlock), 0, M->stream, M->d_views, sb, n,
+                         prandtl);
+    hipLaunchKernelGGL((k_flux_multi<SO, Visc, T>), dim3(per_tiles * kXcds), dim3(kTile), lds, M->stream,
+                       M->d_views, stl, n, per_tiles, parity, irk == 0 ? 1 : 0, beta[irk], visc);
+  }
+}
+
+template <bool SO, class Visc, class T>
+void multi_lim(ma::Multi* M, int parity) {
+  if (M->scheme.limiter == MA_LIMITER_VANALBADA)
+    multi_step_t<SO, Visc, ma::VanAlbadaLimiter, T>(M, parity);
+  else
+    multi_step_t<SO, Visc, ma::VenkatLimiter, T>(M, parity);
+}
+
+template <class T>
+void multi_t(ma::Multi* M, int parity) {
+  const bool so = M->scheme.second_order_space != 0, visc = M->scheme.viscous != 0;
+  if (so && visc) multi_lim<true, ma::NewtonianViscousFlux, T>(M, parity);
+  else if (so) multi_lim<true, ma::NoViscousFlux, T>(M, parity);
+  else if (visc) multi_lim<false, ma::NewtonianViscousFlux, T>(M, parity);
+  else multi_lim<false, ma::NoViscousFlux, T>(M, parity);
+}
+
+}  // namespace
+
+namespace ma {
+
+ma_status multi_step(Multi* M, int parity) {
+  if (M->n == 0) return MA_OK;
+  if (M->fp32)
+    multi_t<float>(M, parity);
+  else
+    multi_t<double>(M, parity);
+  MA_HIP(hipGetLastError());
+  return MA_OK;
+}
+
+void solver_set_progress(ma_solver* s, int steps) {
+  s->cur ^= (steps & 1);
+  s->steps_done += steps;
+}
+
+int solver_cur(const ma_solver* s) { return s->cur; }
 
 }  // namespace ma

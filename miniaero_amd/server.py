@@ -6,9 +6,10 @@
         U = srv.state(r["id"], r["cells"]); srv.release(r["id"])
 
 Cases are planned on host threads, admitted in submission order when a slot
-is free and their device bytes fit in the server's HBM region, and run
-concurrently on their own streams; each result equals the case run alone
-(TimeSolverExplicitRK4) bit for bit.  The reference has no counterpart: it
+is free and their device bytes fit in the server's HBM region, and the
+cases of one scheme are advanced by the same launches (continuous
+batching); each result equals the case run alone (TimeSolverExplicitRK4)
+bit for bit.  The reference has no counterpart: it
 runs one case per process (Main.C:96-141).
 """
 from __future__ import annotations
@@ -23,10 +24,10 @@ from .options import Options
 
 class Server:
     def __init__(self, hbm_budget_bytes: float = 0, max_active: int = 0, setup_threads: int = 0,
-                 steps_per_launch: int = 0):
+                 steps_in_flight: int = 0):
         self._L = _lib.load()
         cfg = _lib.ma_server_config(int(hbm_budget_bytes), int(max_active), int(setup_threads),
-                                    int(steps_per_launch))
+                                    int(steps_in_flight))
         h = ctypes.c_void_p()
         _lib.check(self._L.ma_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

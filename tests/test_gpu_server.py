@@ -1,11 +1,13 @@
-"""Served form: cases admitted while others run.
+"""Served form: cases admitted while others run, the cases of one scheme
+advanced by the same launches (continuous batching).
 
 Bar: every case that ran to its end equals, bit for bit, the same case run
 alone with TimeSolverExplicitRK4 (itself held bit for bit against the oracle
-by the parity tests); requests are admitted in submission order with at most
-max_active on the device; cases that cannot run (bad options, larger than
-the server's region) fail alone without disturbing the others; the region is
-empty again at the end.
+by the parity tests), whatever cases it shared launches with and whenever it
+joined; requests are admitted in submission order with at most max_active on
+the device; cases that cannot run (bad options, larger than the server's
+region) fail alone without disturbing the others; the region is empty again
+at the end.
 """
 import time
 
@@ -39,10 +41,10 @@ def test_served_cases_match_their_solo_runs():
                                  float(np.exp(rng.uniform(np.log(4096), np.log(40000)))),
                                  int(rng.integers(5, 30))))
     cases.append(cases[0].replace(ntimesteps=0))       # initial state only
-    cases.append(cases[1].replace(ntimesteps=13))      # steps not a multiple of a launch
+    cases.append(cases[1].replace(ntimesteps=13))      # odd step count (state in the other buffer)
     bad = cases[2].replace(nx=0)                         # rejected by the mesh generator
     big = served_case(True, 0.5, 0.0, 600000, 5)         # larger than the region below
-    srv = M.Server(hbm_budget_bytes=256 << 20, max_active=3, setup_threads=3, steps_per_launch=4)
+    srv = M.Server(hbm_budget_bytes=256 << 20, max_active=3, setup_threads=3)
     try:
         ids = {}
         for k, o in enumerate(cases[:6]):

@@ -253,7 +253,7 @@ def test_invalid_tile_shape_is_rejected_before_device_use():
 
 def test_server_checks_config_and_device_cleanly():
     with pytest.raises(M.MiniAeroError) as e:
-        M.Server(steps_per_launch=3)
+        M.Server(steps_in_flight=-1)
     assert e.value.status == _lib.MA_ERR_INVALID
     with pytest.raises(M.MiniAeroError) as e:
         M.Server(max_active=-1)
