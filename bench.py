@@ -307,7 +307,7 @@ def bench_served(args, M, stream, copy_gbs) -> dict:
             break
     st0 = srv.stats()
     print(f"[bench] served: {len(stream)} requests, region {st0['region_size'] / 1e9:.1f} GB, "
-          f"max_active {args.max_active or 16}", file=sys.stderr, flush=True)
+          f"max_active {args.max_active or 16}, setup threads {args.setup_threads}", file=sys.stderr, flush=True)
     done, opts_of = {}, {}
     t0 = srv.now()
     last_note = time.time()
@@ -402,9 +402,9 @@ def main() -> int:
                     help="1: pick the flux tile shape by timing candidates at setup (default: the "
                          "workload's setting)")
     ap.add_argument("--cases", type=int, default=200, help="served workload: requests in the stream")
-    ap.add_argument("--max-active", type=int, default=0, help="served workload: cases on the GPU at once")
+    ap.add_argument("--max-active", type=int, default=32, help="served workload: cases on the GPU at once")
     ap.add_argument("--hbm-budget", type=float, default=200e9, help="served workload: device bytes for cases")
-    ap.add_argument("--setup-threads", type=int, default=12,
+    ap.add_argument("--setup-threads", type=int, default=14,
                     help="served workload: host threads generating and planning requests")
     ap.add_argument("--wg-per-cu", type=int, default=None,
                     help="resident workgroups per CU for the gradient/flux kernels (0: one per tile)")
