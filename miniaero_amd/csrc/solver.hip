@@ -1234,7 +1234,7 @@ void merge_layouts(std::vector<Layout>& parts, Layout& M, std::vector<int>& tile
   M.fkind.assign(nf, 0);
   M.fwall.assign(nf, 0.0);
   M.fn.assign((size_t)3 * nf, 0.0);
-  M.ffr.assign((size_t)10 * nf, 0.0);
+  if (!M.derive_frame) M.ffr.assign((size_t)10 * nf, 0.0);
   M.fx.assign((size_t)3 * nf, 0.0);
   M.vol.assign(nc, 0.0);
   M.xc.assign((size_t)3 * nc, 0.0);
@@ -1277,7 +1277,17 @@ void merge_layouts(std::vector<Layout>& parts, Layout& M, std::vector<int>& tile
         M.fn[(size_t)d * nf + g] = L.fn[(size_t)d * lnf + f];
         M.fx[(size_t)d * nf + g] = L.fx[(size_t)d * lnf + f];
       }
-      for (int d = 0; d < 10; ++d) M.ffr[(size_t)d * nf + g] = L.ffr[(size_t)d * lnf + f];
+      if (!M.derive_frame) {
+        if (L.derive_frame) {  // this part stored no frames: they follow from the area vector
+          const double a[3] = {L.fn[f], L.fn[(size_t)lnf + f], L.fn[(size_t)2 * lnf + f]};
+          const ma::FaceFrame fr = ma::frame_from_area(a);
+          const double v[10] = {fr.nn, fr.nu[0], fr.nu[1], fr.nu[2], fr.tu[0],
+                                fr.tu[1], fr.tu[2], fr.bu[0], fr.bu[1], fr.bu[2]};
+          for (int d = 0; d < 10; ++d) M.ffr[(size_t)d * nf + g] = v[d];
+        } else {
+          for (int d = 0; d < 10; ++d) M.ffr[(size_t)d * nf + g] = L.ffr[(size_t)d * lnf + f];
+        }
+      }
       if (f >= lnif) bc_case[bo[k] + (f - lnif)] = k;
     }
     Layout().swap_out(L);  // release the part as soon as it is merged
@@ -1341,27 +1351,40 @@ ma_status plan_layout(const ma_mesh_data* md, const TileShape& shape, Layout& L)
   std::vector<int>& fr = L.fr;
   std::vector<signed char>& fkind = L.fkind;
   fn.assign((size_t)3 * nf, 0.0);
-  ffr.assign((size_t)10 * nf, 0.0);
   fx.assign((size_t)3 * nf, 0.0);
   fwall.assign(nf, 0.0);
   fl.assign(nf, 0);
   fr.assign(nf, 0);
   fkind.assign(nf, 0);
+  // Area vector and centroid of every face; whether every stored tangent
+  // and binormal follow from the area vector as Face.C builds them (then the
+  // kernels rebuild the frame and no frame array is stored).
+  auto derivable = [](const ma_faces& F) {  // in the caller's order (sequential reads)
+    for (int i = 0; i < F.nfaces; ++i) {
+      double t[3], b[3];
+      ma::tangent_binormal(&F.face_normal[3 * i], t, b);
+      for (int d = 0; d < 3; ++d)
+        if (t[d] != F.face_tangent[3 * i + d] || b[d] != F.face_binormal[3 * i + d] ||
+            std::signbit(t[d]) != std::signbit(F.face_tangent[3 * i + d]) ||
+            std::signbit(b[d]) != std::signbit(F.face_binormal[3 * i + d]))
+          return false;
+    }
+    return true;
+  };
+  L.derive_frame = derivable(I);
+  for (int g = 0; g < md->num_boundary_groups && L.derive_frame; ++g)
+    L.derive_frame = derivable(md->boundary_faces[g].faces);
   auto put = [&](int f, const ma_faces& F, int i) {
-    const double* a = &F.face_normal[3 * i];
-    double t[3], b[3];
-    ma::tangent_binormal(a, t, b);
-    for (int d = 0; d < 3; ++d)
-      if (t[d] != F.face_tangent[3 * i + d] || b[d] != F.face_binormal[3 * i + d] ||
-          std::signbit(t[d]) != std::signbit(F.face_tangent[3 * i + d]) ||
-          std::signbit(b[d]) != std::signbit(F.face_binormal[3 * i + d]))
-        L.derive_frame = false;
+    for (int d = 0; d < 3; ++d) {
+      fn[(size_t)d * nf + f] = F.face_normal[3 * i + d];
+      fx[(size_t)d * nf + f] = F.coordinates[3 * i + d];
+    }
+  };
+  auto put_frame = [&](int f, const ma_faces& F, int i) {
     const ma::FaceFrame fr = ma::make_frame(&F.face_normal[3 * i], &F.face_tangent[3 * i],
                                             &F.face_binormal[3 * i]);
     ffr[f] = fr.nn;
     for (int d = 0; d < 3; ++d) {
-      fn[(size_t)d * nf + f] = F.face_normal[3 * i + d];
-      fx[(size_t)d * nf + f] = F.coordinates[3 * i + d];
       ffr[(size_t)(1 + d) * nf + f] = fr.nu[d];
       ffr[(size_t)(4 + d) * nf + f] = fr.tu[d];
       ffr[(size_t)(7 + d) * nf + f] = fr.bu[d];
@@ -1389,6 +1412,15 @@ ma_status plan_layout(const ma_mesh_data* md, const TileShape& shape, Layout& L)
           dist += std::pow(G.faces.coordinates[3 * i + d] - md->cell_coordinates[3 * c + d], 2);
         fwall[f] = 1.0 / std::sqrt(dist);
       }
+    }
+  }
+  if (!L.derive_frame) {  // stored frames (|n|, unit n, t, b) from the caller's vectors
+    ffr.assign((size_t)10 * nf, 0.0);
+    for (int g = 0; g < nif; ++g) put_frame(g, I, dface[g]);
+    int f = nif;
+    for (int g = 0; g < md->num_boundary_groups; ++g) {
+      const ma_bc_group& G = md->boundary_faces[g];
+      for (int i = 0; i < G.faces.nfaces; ++i, ++f) put_frame(f, G.faces, i);
     }
   }
 

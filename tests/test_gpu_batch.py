@@ -210,3 +210,31 @@ def test_batch_residual_norms_are_per_case():
     for k in range(3):
         lo, hi = b.offsets[k], b.offsets[k + 1]
         assert np.allclose(n[k], np.linalg.norm(R[lo:hi], axis=0), rtol=1e-13, atol=0.0)
+
+
+def test_batch_mixing_stored_and_derived_frames():
+    """A batch with one mesh whose frames must be stored (tangents scaled by
+    2) stores frames for every case; each case still equals its solo run."""
+    kw = VANALBADA_CASES[3]
+    md, opts = make_case(kw)
+    def scaled(F):
+        return M.Faces(F.coordinates, F.face_normal, 2.0 * F.face_tangent, 2.0 * F.face_binormal,
+                       F.face_cell_conn, F.cell_flux_index, F.permute_vector)
+    md2 = M.MeshData(md.num_owned_cells, md.cell_coordinates, md.cell_volumes,
+                     scaled(md.internal_faces), [(n, scaled(F)) for n, F in md.boundary_faces])
+    cases = [(md, opts), (md2, opts), make_case(VANALBADA_CASES[0])]
+    b = M.BatchSolver(cases)
+    try:
+        b.initialize()
+        b.step(6)
+        got = [b.case_state(k) for k in range(3)]
+    finally:
+        b.close()
+    for k, (mesh, o) in enumerate(cases):
+        s = M.TimeSolverExplicitRK4(mesh, o)
+        try:
+            s.initialize()
+            s.step(6)
+            assert np.array_equal(got[k], s.state()), f"case {k}"
+        finally:
+            s.close()

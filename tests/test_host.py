@@ -277,3 +277,20 @@ def test_served_stream_is_seeded_and_shaped():
     assert 0.35 < visc.mean() < 0.65
     steps = np.array([o.ntimesteps for _, o in a])
     assert steps.min() >= 50 and steps.max() <= 500
+
+
+def test_batch_plan_with_stored_and_derived_frames():
+    """One mesh whose tangents do not follow Face.C (scaled by 2) makes the
+    whole batch store face frames; the other parts' frames are then rebuilt
+    from their area vectors on the host."""
+    from miniaero_amd.mesh import plan_batch
+    md = M.Parallel3DMesh(6, 5, 4, 2.0, 2.0, 1.0, 2, 30.0).fillMeshData()
+    def scaled(F):
+        return M.Faces(F.coordinates, F.face_normal, 2.0 * F.face_tangent, 2.0 * F.face_binormal,
+                       F.face_cell_conn, F.cell_flux_index, F.permute_vector)
+    md2 = M.MeshData(md.num_owned_cells, md.cell_coordinates, md.cell_volumes,
+                     scaled(md.internal_faces), [(n, scaled(F)) for n, F in md.boundary_faces])
+    assert plan_batch([md, md])["derived_frames"]
+    p = plan_batch([md, md2, md])
+    assert not p["derived_frames"]
+    assert p["tiles"] == 3 * md.plan()["tiles"]
