@@ -407,7 +407,7 @@ def main() -> int:
     ap.add_argument("--setup-threads", type=int, default=14,
                     help="served workload: host threads generating and planning requests")
     ap.add_argument("--wg-per-cu", type=int, default=None,
-                    help="resident workgroups per CU for the gradient/flux kernels (0: one per tile)")
+                    help="resident workgroups per CU for the flux kernel (0: one per tile)")
     ap.add_argument("--copy-ceiling", action="store_true",
                     help="only measure the plain copy kernel (1 GiB, read + write) and exit")
     args = ap.parse_args()

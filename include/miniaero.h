@@ -81,8 +81,8 @@ typedef struct ma_options {
    * lattice always use consecutive cells.  Results do not depend on it. */
   int tile_shape[3];
   int autotune;              /* 1: time candidate tile shapes and workgroups_per_cu at create, keep the fastest */
-  int workgroups_per_cu;     /* gradient and flux kernels: resident workgroups per CU walking the
-                              * tiles (0 = one workgroup per tile); results do not depend on it */
+  int workgroups_per_cu;     /* flux kernel: resident workgroups per CU walking the tiles
+                              * (0 = one workgroup per tile); results do not depend on it */
 } ma_options;
 
 /* One Faces<Device> (Faces.h:42-71), as host arrays. */

@@ -76,7 +76,9 @@ struct DevMesh {
   const double* __restrict__ fn;   // 3 planes of nf: area vector
   const double* __restrict__ ffr;  // 10 planes of nf: |n|, unit n, unit t, unit b (or null)
   int derive_frame;                // 1: rebuild the frame from fn (ffr unused)
-  const double* __restrict__ fx;
+  // 6 planes of nf in the arithmetic type: T(x_f - x_left) (planes 0-2) and
+  // T(x_f - x_right) (3-5, internal faces), from FP64 centroids
+  const void* __restrict__ fdx;
   const int* __restrict__ fl;
   const int* __restrict__ fr;
   const signed char* __restrict__ fkind;
@@ -326,8 +328,6 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const T* __restrict_
     }
   if (!kSecondOrder) return;
 
-  double xcc[3];
-  load3(m.xc, nc, c, xcc);
   T lim[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) lim[k] = T(1.0);
@@ -336,11 +336,11 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const T* __restrict_
     const int e = ent[s];
     if (e < 0) break;
     const int f = e >> 1;
-    // displacement from absolute FP64 coordinates, then the arithmetic type
+    // displacement from the FP64 centroids, rounded to the arithmetic type
     T disp[3], dist = 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      disp[d] = T(m.fx[(size_t)d * nf + f] - xcc[d]);
+      disp[d] = static_cast<const T*>(m.fdx)[(size_t)((e & 1) * 3 + d) * nf + f];
       dist += disp[d] * disp[d];
     }
 #pragma unroll
@@ -361,26 +361,26 @@ template <bool kSecondOrder, class Limiter, class T>
 __global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const T* __restrict__ V,
                                                  T* __restrict__ grad, T* __restrict__ phi,
                                                  int per_xcd) {
-  // XCD-aware: each XCD takes a contiguous run of blocks, walked with stride
-  // gridDim.x / kXcds when the grid is smaller than the block count (see k_flux)
-  const int xcd = blockIdx.x % kXcds, stride = gridDim.x / kXcds;
-  for (int j = blockIdx.x / kXcds; j < per_xcd; j += stride) {
-    const int c = (xcd * per_xcd + j) * kBlock + threadIdx.x;
-    if (c < m.nc) grad_cell<kSecondOrder, Limiter, T>(m, V, grad, phi, c);
-  }
+  // XCD-aware: each XCD takes a contiguous run of blocks (see k_flux)
+  const int b = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  const int c = b * kBlock + threadIdx.x;
+  if (c < m.nc) grad_cell<kSecondOrder, Limiter, T>(m, V, grad, phi, c);
 }
 
 // Reconstructed (second order) primitive state of cell `c` at face centroid xf:
 // compute_face_flux's extrapolation, Flux.h:109-132.  The displacement is
 // taken from the FP64 coordinates, then carried in the arithmetic type T.
+// Second-order extrapolation of cell c's primitives to face f (Flux.h:107-130):
+// v += phi * (x_f - x_c) . grad, with x_f - x_c from the displacement planes
+// (side 0: the face's left cell, 1: its right cell).
 template <class T>
 __device__ __forceinline__ void reconstruct(const DevMesh& m, const T* __restrict__ grad,
-                                            const T* __restrict__ phi, int c, const double* xf,
-                                            T* v) {
+                                            const T* __restrict__ phi, int c, int f, int side, T* v) {
   const int nc = m.nc;
+  const T* dxp = static_cast<const T*>(m.fdx) + (size_t)side * 3 * m.nf + f;
   T dx[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) dx[d] = T(xf[d] - m.xc[(size_t)d * nc + c]);
+  for (int d = 0; d < 3; ++d) dx[d] = dxp[(size_t)d * m.nf];
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
     T t = 0;
@@ -407,10 +407,8 @@ __device__ __forceinline__ void interior_flux(const DevMesh& m, const T* __restr
   load5(V, nc, Rr, vr);
   const ma::FaceFrameT<T> fr = load_frame<T>(m, f);
   if (kSecondOrder) {
-    double xf[3];
-    load3(m.fx, m.nf, f, xf);
-    reconstruct(m, grad, phi, L, xf, vl);
-    reconstruct(m, grad, phi, Rr, xf, vr);
+    reconstruct(m, grad, phi, L, f, 0, vl);
+    reconstruct(m, grad, phi, Rr, f, 1, vr);
   }
   const ma::RoeFlux roe;
   roe.flux_frame(vl, vr, F, fr);
@@ -915,7 +913,7 @@ struct ma_solver {
   int graph_steps = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   TileShape shape = kDefaultShape;  // flux tile shape in use
-  int wg_per_cu = 0;                 // resident workgroups per CU of k_grad/k_flux (0: full grid)
+  int wg_per_cu = 0;                 // resident workgroups per CU of k_flux (0: full grid)
   int cus = 256;                     // compute units of the device
   bool all_lattice = true;
   double tuned_ms = 0.0;  // autotuner: measured ms per step per million cells of the chosen shape
@@ -1532,6 +1530,30 @@ ma_status check_mesh(const ma_mesh_data* md, const char* who) {
   return MA_OK;
 }
 
+// Uploads the 6 displacement planes T(x_f - x_left), T(x_f - x_right) of
+// every face (right planes zero for boundary faces).
+template <class T>
+ma_status upload_disp(ma_solver* s, const Layout& L, void** out) {
+  const int nf = L.nf, nc = L.nc;
+  std::vector<T> h;
+  if (!s->counting) {
+    h.assign((size_t)6 * nf, T(0));
+    for (int f = 0; f < nf; ++f)
+      for (int side = 0; side < 2; ++side) {
+        const int c = side ? L.fr[f] : L.fl[f];
+        if (c < 0) continue;
+        for (int d = 0; d < 3; ++d)
+          h[(size_t)(side * 3 + d) * nf + f] = T(L.fx[(size_t)d * nf + f] - L.xc[(size_t)d * nc + c]);
+      }
+  } else {
+    h.resize(1);
+  }
+  T* p = nullptr;
+  ma_status st = s->counting ? s->alloc(&p, (size_t)6 * nf) : upload(s, &p, h);
+  *out = p;
+  return st;
+}
+
 // Layout of a run over ncases meshes (concatenated when there are several),
 // checked, with the case of every tile, boundary face and device cell.
 struct RunPlan {
@@ -1627,13 +1649,13 @@ ma_status build_planned(ma_solver* s, RunPlan& P) {
   m.nc = nc;
   m.nif = nif;
   m.nf = nf;
-  double *dvol, *dxc, *dfn, *dffr = nullptr, *dfx, *dfw;
+  double *dvol, *dxc, *dfn, *dffr = nullptr, *dfw;
   int *dslot, *dfl, *dfr, *dorg;
   signed char* dkind;
   if ((st = upload(s, &dvol, L.vol)) || (st = upload(s, &dxc, L.xc)) || (st = upload(s, &dslot, L.slot)) ||
       (st = upload(s, &dfn, L.fn)) || (!L.derive_frame && (st = upload(s, &dffr, L.ffr))) ||
       (st = upload(s, &dorg, L.org_of)) ||
-      (st = upload(s, &dfx, L.fx)) || (st = upload(s, &dfl, L.fl)) || (st = upload(s, &dfr, L.fr)) ||
+      (st = upload(s, &dfl, L.fl)) || (st = upload(s, &dfr, L.fr)) ||
       (st = upload(s, &dkind, L.fkind)) || (st = upload(s, &dfw, L.fwall)))
     return st;
   m.vol = dvol;
@@ -1643,7 +1665,10 @@ ma_status build_planned(ma_solver* s, RunPlan& P) {
   m.fn = dfn;
   m.ffr = L.derive_frame ? nullptr : dffr;
   m.derive_frame = L.derive_frame ? 1 : 0;
-  m.fx = dfx;
+  // displacement planes T(x_f - x_cell) from the FP64 centroids
+  void* dfdx = nullptr;
+  if ((st = s->fp32 ? upload_disp<float>(s, L, &dfdx) : upload_disp<double>(s, L, &dfdx))) return st;
+  m.fdx = dfdx;
   m.fl = dfl;
   m.fr = dfr;
   m.fkind = dkind;
@@ -1740,7 +1765,7 @@ void run_stage(ma_solver* s, int irk) {
   if (SO || Visc::isViscous) {
     s->launch(SO ? "k_grad_limiter" : "k_grad", [&] {
       const int per = (nb + kXcds - 1) / kXcds;
-      hipLaunchKernelGGL((k_grad<SO, Lim, T>), dim3(resident(s, per) * kXcds), dim3(kBlock), 0, s->stream, m,
+      hipLaunchKernelGGL((k_grad<SO, Lim, T>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, m,
                          static_cast<const T*>(s->V), static_cast<T*>(s->grad), static_cast<T*>(s->phi),
                          per);
     });

@@ -41,7 +41,7 @@ class Options:
     precision: int = 0        # 0 FP64, 1 FP32 flux/gradient/limiter arithmetic
     tile_shape: tuple = (0, 0, 0)   # flux tile brick (i, j, k); zeros = 4x8x8, (-1, ..) = consecutive
     autotune: int = 0         # 1: pick the fastest tile shape / resident workgroups at creation
-    workgroups_per_cu: int = 0  # gradient/flux kernels: resident workgroups per CU (0: one per tile)
+    workgroups_per_cu: int = 0  # flux kernel: resident workgroups per CU (0: one per tile)
 
     def read_options_file(self, path: str = "miniaero.inp") -> "Options":
         """Options::read_options_file (Options.h:73-101): nine whitespace
