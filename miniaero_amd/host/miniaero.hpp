@@ -159,6 +159,7 @@ class TimeSolverExplicitRK4 {
     if (opt_.output_results) write_results(rank);
   }
 
+  void initialize() { check(ma_solver_initialize(s_)); }
   void step(int n) { check(ma_solver_step(s_, n)); }
   std::vector<double> state() {
     std::vector<double> U((size_t)mesh_.num_owned_cells() * 5);
@@ -186,6 +187,82 @@ class TimeSolverExplicitRK4 {
   MeshData& mesh_;
   ma_options opt_;
   ma_solver* s_ = nullptr;
+};
+
+// Independent cases advanced by the same launches (ma_batch_create); no
+// reference counterpart (the reference runs one case per process).  Each
+// case's result equals a TimeSolverExplicitRK4 run of it alone, bit for bit.
+class BatchSolver {
+ public:
+  BatchSolver(const std::vector<MeshData*>& meshes, const std::vector<ma_options>& options) {
+    if (meshes.size() != options.size() || meshes.empty()) throw Error(MA_ERR_INVALID, "BatchSolver: sizes");
+    std::vector<const ma_mesh_data*> views;
+    for (MeshData* m : meshes) {
+      views.push_back(&m->view());
+      cells_.push_back(m->num_owned_cells());
+    }
+    check(ma_batch_create((int)views.size(), views.data(), options.data(), &s_));
+  }
+  ~BatchSolver() { ma_solver_destroy(s_); }
+  BatchSolver(const BatchSolver&) = delete;
+  BatchSolver& operator=(const BatchSolver&) = delete;
+
+  void initialize() { check(ma_solver_initialize(s_)); }
+  void step(int n) { check(ma_solver_step(s_, n)); }
+  void set_graph(int steps_per_graph) { check(ma_solver_set_graph(s_, steps_per_graph)); }
+  int cases() const { return (int)cells_.size(); }
+  std::vector<double> case_state(int k) {
+    std::vector<double> U((size_t)cells_.at(k) * 5);
+    check(ma_batch_get_case_state(s_, k, U.data()));
+    return U;
+  }
+  double case_time(int k) {
+    double t = 0, dt = 0;
+    check(ma_batch_get_case_time(s_, k, &t, &dt));
+    return t;
+  }
+  ma_solver* handle() { return s_; }
+
+ private:
+  std::vector<int> cells_;
+  ma_solver* s_ = nullptr;
+};
+
+// Request loop over independent cases (ma_server_*): submit options, poll
+// finished cases, read their states.  No reference counterpart.
+class Server {
+ public:
+  explicit Server(long long hbm_budget_bytes = 0, int max_active = 0, int setup_threads = 0,
+                  int steps_in_flight = 0) {
+    ma_server_config cfg{hbm_budget_bytes, max_active, setup_threads, steps_in_flight};
+    check(ma_server_create(&cfg, &s_));
+  }
+  ~Server() { ma_server_destroy(s_); }
+  Server(const Server&) = delete;
+  Server& operator=(const Server&) = delete;
+
+  long long submit(const ma_options& options) {
+    long long id = 0;
+    check(ma_server_submit(s_, &options, &id));
+    return id;
+  }
+  std::vector<ma_case_result> poll(double timeout_s, int max_results = 64) {
+    std::vector<ma_case_result> out((size_t)max_results);
+    const int n = ma_server_poll(s_, out.data(), max_results, timeout_s);
+    if (n < 0) throw Error(MA_ERR_INVALID, ma_last_error());
+    out.resize((size_t)n);
+    return out;
+  }
+  std::vector<double> state(const ma_case_result& r) {
+    std::vector<double> U((size_t)r.cells * 5);
+    check(ma_server_get_state(s_, r.id, U.data()));
+    return U;
+  }
+  void release(long long id) { check(ma_server_release(s_, id)); }
+  double now() { return ma_server_now(s_); }
+
+ private:
+  ma_server* s_ = nullptr;
 };
 
 }  // namespace miniaero

@@ -238,3 +238,15 @@ def test_batch_mixing_stored_and_derived_frames():
             assert np.array_equal(got[k], s.state()), f"case {k}"
         finally:
             s.close()
+
+
+def test_cpp_facade_batched_and_served_cases(tmp_path):
+    """bin/many_cases: the C++ facade's BatchSolver and Server against solo
+    TimeSolverExplicitRK4 runs of the same cases, bit for bit."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "miniaero_amd", "bin", "many_cases")
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=240)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK")

@@ -294,3 +294,13 @@ def test_batch_plan_with_stored_and_derived_frames():
     p = plan_batch([md, md2, md])
     assert not p["derived_frames"]
     assert p["tiles"] == 3 * md.plan()["tiles"]
+
+
+def test_cpp_many_cases_program_reports_missing_device(tmp_path):
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    import subprocess
+    exe = os.path.join(ROOT, "miniaero_amd", "bin", "many_cases")
+    assert os.path.exists(exe), "bin/many_cases not built"
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "no HIP device" in r.stderr
