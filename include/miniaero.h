@@ -254,7 +254,9 @@ ma_status ma_server_submit(ma_server* srv, const ma_options* opts, long long* id
  * to timeout_s seconds for the first; returns the count (-1 on error). */
 int ma_server_poll(ma_server* srv, ma_case_result* out, int max, double timeout_s);
 /* Final state of a finished case (cells x 5 conserved values, the generated
- * mesh's cell order); kept on the host until ma_server_release. */
+ * mesh's cell order); kept on the host until ma_server_release.  A case whose
+ * final state holds non-finite values finishes with MA_ERR_NONFINITE; its
+ * state is still copied out and the call returns MA_ERR_NONFINITE. */
 ma_status ma_server_get_state(ma_server* srv, long long id, double* U);
 ma_status ma_server_release(ma_server* srv, long long id);
 /* stats[0] submitted, [1] waiting (planning or for admission), [2] active,

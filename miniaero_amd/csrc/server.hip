@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -285,6 +286,13 @@ void complete(ma_server* S, Request* r) {
       r->status = st;
       r->error = ma::last_error();
       r->U.clear();
+    } else {
+      long long bad = 0;
+      for (double x : r->U) bad += !std::isfinite(x);
+      if (bad) {  // the state is kept for inspection
+        r->status = MA_ERR_NONFINITE;
+        r->error = std::to_string(bad) + " non-finite values in the final state";
+      }
     }
   }
   retire(S, r);
@@ -554,9 +562,10 @@ extern "C" ma_status ma_server_get_state(ma_server* S, long long id, double* U) 
   if (it == S->all.end()) return ma::fail(MA_ERR_INVALID, "ma_server_get_state: unknown or released id");
   Request* r = it->second.get();
   if (!r->finished) return ma::fail(MA_ERR_INVALID, "ma_server_get_state: case not finished");
-  if (r->status != MA_OK || r->U.empty())
+  if (r->U.empty() || (r->status != MA_OK && r->status != MA_ERR_NONFINITE))
     return ma::fail(r->status ? (ma_status)r->status : MA_ERR_INVALID, "case failed: " + r->error);
   std::memcpy(U, r->U.data(), r->U.size() * sizeof(double));
+  if (r->status == MA_ERR_NONFINITE) return ma::fail(MA_ERR_NONFINITE, r->error);  // U is filled
   return MA_OK;
 }
 
