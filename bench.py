@@ -56,6 +56,21 @@ def algorithmic_bytes(kernel: str, nc: int, nif: int, nbf: int, second_order: bo
     return 0.0
 
 
+def path_bytes_per_step(nc: int, nif: int, nbf: int, second_order: bool, viscous: bool) -> float:
+    """Compulsory HBM bytes of one RK4 step of the whole path (4 stages):
+    the per-stage figures of the fused target design -- 824 N_c + 168 F
+    (second order, inviscid or viscous), 536 N_c + 112 F (first-order
+    viscous), 208 N_c + 80 F (first-order inviscid); FP64 sizes."""
+    F = nif + nbf
+    if second_order:
+        per = 824.0 * nc + 168.0 * F
+    elif viscous:
+        per = 536.0 * nc + 112.0 * F
+    else:
+        per = 208.0 * nc + 80.0 * F
+    return 4.0 * per
+
+
 def _oracle_rate(cases, steps: int):
     """Cell-stage updates/s of the CPU oracle over the given options, run one
     after another for `steps` RK4 steps each."""
@@ -201,6 +216,11 @@ def bench_single(args, M, opts, copy_gbs) -> dict:
     so = bool(opts.second_order_space)
     roof = roofline(solver, args.profile_steps, nc, nif, nbf, so, opts.precision != 0,
                     lambda k: load_traffic(args.workload, k), copy_gbs)
+    if opts.precision == 0:
+        pb = path_bytes_per_step(nc, nif, nbf, so, bool(opts.viscous))
+        path_gbs = pb / (elapsed / args.steps) / 1e9
+        roof["path"] = {"algorithmic_bytes_per_step": pb, "achieved_GBps": round(path_gbs, 1),
+                        "frac": round(path_gbs / HBM_PEAK_GBS, 4)}
     cpu = None
     if not args.no_cpu_baseline:
         try:
