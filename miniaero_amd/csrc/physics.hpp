@@ -68,11 +68,12 @@ struct FaceFrameT {
 };
 using FaceFrame = FaceFrameT<double>;
 
-MA_HD FaceFrame make_frame(const double* fn, const double* ft, const double* fb) {
-  FaceFrame f;
-  const double nn = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
-  const double tn = sqrt(ft[0] * ft[0] + ft[1] * ft[1] + ft[2] * ft[2]);
-  const double bn = sqrt(fb[0] * fb[0] + fb[1] * fb[1] + fb[2] * fb[2]);
+template <class T>
+MA_HD FaceFrameT<T> make_frame_t(const T* fn, const T* ft, const T* fb) {
+  FaceFrameT<T> f;
+  const T nn = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+  const T tn = sqrt(ft[0] * ft[0] + ft[1] * ft[1] + ft[2] * ft[2]);
+  const T bn = sqrt(fb[0] * fb[0] + fb[1] * fb[1] + fb[2] * fb[2]);
   for (int d = 0; d < 3; ++d) {
     f.n[d] = fn[d];
     f.nu[d] = fn[d] / nn;
@@ -83,30 +84,36 @@ MA_HD FaceFrame make_frame(const double* fn, const double* ft, const double* fb)
   return f;
 }
 
+MA_HD FaceFrame make_frame(const double* fn, const double* ft, const double* fb) {
+  return make_frame_t<double>(fn, ft, fb);
+}
+
 // The tangent and binormal the reference attaches to a face are functions of
 // its area vector a alone (Face.C:80-96): i1 = first index of the largest
 // |a_i|, t[i1+1] = 0, t[i1] = a[i1+2]/den, t[i1+2] = -a[i1]/den with
 // den = sqrt(a[i1]^2 + a[i1+2]^2) (indices mod 3), b = a x t.  When a mesh's
 // t and b are exactly these, the frame can be rebuilt from a on the fly with
-// the same operations, which is cheaper than reading it.
-MA_HD void tangent_binormal(const double* a, double* t, double* b) {
-  const double m0 = fabs(a[0]), m1 = fabs(a[1]), m2 = fabs(a[2]);
+// the same operations, which is cheaper than reading it.  In FP64 this gives
+// the reference's bits; the FP32-arithmetic form builds it in FP32.
+template <class T>
+MA_HD void tangent_binormal_t(const T* a, T* t, T* b) {
+  const T m0 = fabs(a[0]), m1 = fabs(a[1]), m2 = fabs(a[2]);
   int i1 = 0;
   if (m1 > m0) i1 = 1;
   if (m2 > (i1 == 0 ? m0 : m1)) i1 = 2;
   if (i1 == 0) {
-    const double den = sqrt(a[0] * a[0] + a[2] * a[2]);
+    const T den = sqrt(a[0] * a[0] + a[2] * a[2]);
     t[0] = a[2] / den;
-    t[1] = 0.0;
+    t[1] = T(0.0);
     t[2] = -a[0] / den;
   } else if (i1 == 1) {
-    const double den = sqrt(a[1] * a[1] + a[0] * a[0]);
+    const T den = sqrt(a[1] * a[1] + a[0] * a[0]);
     t[0] = -a[1] / den;
     t[1] = a[0] / den;
-    t[2] = 0.0;
+    t[2] = T(0.0);
   } else {
-    const double den = sqrt(a[2] * a[2] + a[1] * a[1]);
-    t[0] = 0.0;
+    const T den = sqrt(a[2] * a[2] + a[1] * a[1]);
+    t[0] = T(0.0);
     t[1] = -a[2] / den;
     t[2] = a[1] / den;
   }
@@ -115,11 +122,16 @@ MA_HD void tangent_binormal(const double* a, double* t, double* b) {
   b[2] = a[0] * t[1] - t[0] * a[1];
 }
 
-MA_HD FaceFrame frame_from_area(const double* a) {
-  double t[3], b[3];
-  tangent_binormal(a, t, b);
-  return make_frame(a, t, b);
+MA_HD void tangent_binormal(const double* a, double* t, double* b) { tangent_binormal_t<double>(a, t, b); }
+
+template <class T>
+MA_HD FaceFrameT<T> frame_from_area_t(const T* a) {
+  T t[3], b[3];
+  tangent_binormal_t<T>(a, t, b);
+  return make_frame_t<T>(a, t, b);
 }
+
+MA_HD FaceFrame frame_from_area(const double* a) { return frame_from_area_t<double>(a); }
 
 struct RoeFlux {
   // Roe_Flux.h:50-54 signature: builds the face frame, then the flux.

@@ -70,10 +70,9 @@ struct DevMesh {
   int nif;   // internal faces
   int nf;    // all faces
   const double* __restrict__ vol;
-  const double* __restrict__ xc;   // 3 planes of nc
-  const int* __restrict__ org;     // device cell -> caller's cell index
   const int* __restrict__ slot;    // kMaxSlots planes of nc; entry = face*2 + side, -1 empty
   const double* __restrict__ fn;   // 3 planes of nf: area vector
+  const float* __restrict__ fn32;  // the same in FP32 (FP32-arithmetic form only)
   const double* __restrict__ ffr;  // 10 planes of nf: |n|, unit n, unit t, unit b (or null)
   int derive_frame;                // 1: rebuild the frame from fn (ffr unused)
   // 6 planes of nf in the arithmetic type: T(x_f - x_left) (planes 0-2) and
@@ -115,24 +114,36 @@ __device__ __forceinline__ void load3(const S* __restrict__ a, int n, int i, T* 
   for (int k = 0; k < 3; ++k) v[k] = T(a[(size_t)k * n + i]);
 }
 
+// Area vectors in the arithmetic type.
+template <class T>
+__device__ __forceinline__ const T* area(const DevMesh& m);
+template <>
+__device__ __forceinline__ const double* area<double>(const DevMesh& m) {
+  return m.fn;
+}
+template <>
+__device__ __forceinline__ const float* area<float>(const DevMesh& m) {
+  return m.fn32;
+}
+
 template <class T>
 __device__ __forceinline__ ma::FaceFrameT<T> load_frame(const DevMesh& m, int f) {
   ma::FaceFrameT<T> fr;
   if (m.derive_frame) {
-    double a[3];
-    load3(m.fn, m.nf, f, a);
-    const ma::FaceFrame d = ma::frame_from_area(a);
-    fr.nn = T(d.nn);
+    T a[3];
+    load3(area<T>(m), m.nf, f, a);
+    const ma::FaceFrameT<T> d = ma::frame_from_area_t<T>(a);
+    fr.nn = d.nn;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      fr.n[k] = T(d.n[k]);
-      fr.nu[k] = T(d.nu[k]);
-      fr.tu[k] = T(d.tu[k]);
-      fr.bu[k] = T(d.bu[k]);
+      fr.n[k] = d.n[k];
+      fr.nu[k] = d.nu[k];
+      fr.tu[k] = d.tu[k];
+      fr.bu[k] = d.bu[k];
     }
     return fr;
   }
-  load3(m.fn, m.nf, f, fr.n);
+  load3(area<T>(m), m.nf, f, fr.n);
   const double* p = m.ffr;
   const size_t nf = m.nf;
   fr.nn = T(p[f]);
@@ -173,14 +184,15 @@ __device__ __forceinline__ double unif(unsigned long long seed, long long cell, 
   return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_init(DevMesh m, const IcParams* __restrict__ cases,
-                                                 const int* __restrict__ cell_case,
-                                                 double* __restrict__ U) {
+// xc: cell centroids (3 planes of nc); org: device cell -> the caller's index.
+__global__ __launch_bounds__(kBlock) void k_init(DevMesh m, const double* __restrict__ xc,
+                                                 const int* __restrict__ org, const IcParams* __restrict__ cases,
+                                                 const int* __restrict__ cell_case, double* __restrict__ U) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= m.nc) return;
   const IcParams& p = cases[cell_case[c]];
   const double Rgas = 287.05, gamma = 1.4;
-  const double x = m.xc[c];
+  const double x = xc[c];
   double u[5];
   if (p.kind == MA_IC_SOD) {
     const double Cv = Rgas / (gamma - 1.0);
@@ -196,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_init(DevMesh m, const IcParams* __re
     ma::primitives(p.inflow, V);
     const double speed = sqrt(V[1] * V[1] + V[2] * V[2] + V[3] * V[3]);
     const double pr = V[0] * Rgas * V[4];
-    const int oc = m.org[c] - p.cell0;  // the draw is keyed by the case's own cell index
+    const int oc = org[c] - p.cell0;  // the draw is keyed by the case's own cell index
     const double rho = V[0] * (1.0 + p.perturb * unif(p.seed, oc, 0));
     const double uu = V[1] + p.perturb * speed * unif(p.seed, oc, 1);
     const double vv = V[2] + p.perturb * speed * unif(p.seed, oc, 2);
@@ -278,7 +290,7 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const T* __restrict_
     if (e < 0) break;
     const int f = e >> 1;
     T n[3];
-    load3(m.fn, nf, f, n);
+    load3(area<T>(m), nf, f, n);
     if (f < m.nif) {
       const int side = e & 1;
       const int nb = side ? m.fl[f] : m.fr[f];
@@ -906,6 +918,8 @@ struct ma_solver {
   Inflow* inflow_dev = nullptr;
   unsigned long long* counter = nullptr;
   double *norm_partial = nullptr, *norm_out = nullptr;  // per tile x 5, per case x 5
+  const double* xc = nullptr;  // cell centroids (3 planes), for the initial condition
+  const int* org = nullptr;    // device cell -> caller's cell index
   int cur = 0;  // U[cur] holds sol_n
   long long steps_done = 0;
   // HIP graphs of graph_steps whole RK4 steps (an even count, so U[cur] is
@@ -1659,10 +1673,22 @@ ma_status build_planned(ma_solver* s, RunPlan& P) {
       (st = upload(s, &dkind, L.fkind)) || (st = upload(s, &dfw, L.fwall)))
     return st;
   m.vol = dvol;
-  m.xc = dxc;
+  s->xc = dxc;
   m.slot = dslot;
-  m.org = dorg;
+  s->org = dorg;
   m.fn = dfn;
+  m.fn32 = nullptr;
+  if (s->fp32) {  // the area vector in FP32 for the FP32-arithmetic kernels
+    float* dfn32 = nullptr;
+    if (s->counting) {
+      st = s->alloc(&dfn32, (size_t)3 * nf);
+    } else {
+      std::vector<float> h(L.fn.begin(), L.fn.end());
+      st = upload(s, &dfn32, h);
+    }
+    if (st) return st;
+    m.fn32 = dfn32;
+  }
   m.ffr = L.derive_frame ? nullptr : dffr;
   m.derive_frame = L.derive_frame ? 1 : 0;
   // displacement planes T(x_f - x_cell) from the FP64 centroids
@@ -2091,8 +2117,8 @@ extern "C" ma_status ma_batch_create(int ncases, const ma_mesh_data* const* mesh
 extern "C" ma_status ma_solver_initialize(ma_solver* s) {
   if (!s) return ma::fail(MA_ERR_INVALID, "null solver");
   s->cur = 0;
-  hipLaunchKernelGGL(k_init, dim3(nblocks(s->m.nc)), dim3(kBlock), 0, s->stream, s->m, s->ic_dev, s->cell_case,
-                     s->U[0]);
+  hipLaunchKernelGGL(k_init, dim3(nblocks(s->m.nc)), dim3(kBlock), 0, s->stream, s->m, s->xc, s->org, s->ic_dev,
+                     s->cell_case, s->U[0]);
   MA_HIP(hipGetLastError());
   const size_t n5 = (size_t)5 * s->m.nc;
   MA_HIP(hipMemsetAsync(s->R, 0, n5 * sizeof(double), s->stream));
