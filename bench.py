@@ -40,11 +40,15 @@ def algorithmic_bytes(kernel: str, nc: int, nif: int, nbf: int, second_order: bo
     connectivity is 8 B per face."""
     F = nif + nbf
     if kernel == "k_flux":
-        # V, grad, phi, x_c, vol, U_acc read; U_acc, R written per cell;
-        # left/right, area/tangent/binormal, x_f per face
+        # V, grad, phi, x_c, vol, U_acc read and U_acc written per cell;
+        # stages 0-2 also read U_n (the same array as U_acc's input on stage
+        # 0) and write the next stage's V, stage 3 writes R: averaged over
+        # the four launches of a step, 110 + 3.75 t bytes per cell; per face
+        # left/right, area/tangent/binormal, x_f
+        per_cell = 110.0 + 3.75 * t
         if second_order:
-            return (25 * t + 4 * g + 120.0) * nc + (8 + 12 * g) * F
-        return (5 * t + g + 120.0) * nc + (8 + 9 * g) * F
+            return (25 * t + 4 * g + per_cell) * nc + (8 + 12 * g) * F
+        return (5 * t + g + per_cell) * nc + (8 + 9 * g) * F
     if kernel in ("k_grad", "k_grad_limiter"):
         # V, vol, x_c read; grad (+ phi) written; left/right, area, x_f per face
         vals = 25 if kernel == "k_grad_limiter" else 20

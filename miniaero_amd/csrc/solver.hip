@@ -8,7 +8,8 @@
 // launches and fences per second-order stage -- with three cell-centric
 // kernels and no atomics:
 //
-//   k_prim   U_s = U_n + alpha_k R;  V = primitives(U_s)           (per cell)
+//   k_prim   U_s = U_n + alpha_k R;  V = primitives(U_s)           (per cell, stage 0;
+//            stages 1-3 get V from the previous stage's k_flux)
 //   k_grad   Green-Gauss gradient, stencil min/max and the limiter  (per cell)
 //            of every cell, from V of the cell and its face neighbours
 //   k_bc     boundary-face contributions (Roe of the ghost state    (per boundary face)
@@ -16,7 +17,9 @@
 //   k_flux   per tile of 256 cells: every internal face touching    (per cell tile)
 //            the tile evaluated once into LDS (Roe [+ reconstruction]
 //            [+ viscous]), then per cell: sum of face and boundary
-//            contributions, residual R = dt/V sum, U_acc += beta_k R
+//            contributions, residual R = dt/V sum, U_acc += beta_k R,
+//            and (stages 0-2) the next stage's V from U_n + alpha_k+1 R
+//            into the other V buffer (R itself is stored on stage 3 only)
 //
 // Determinism and parity.  The reference scatters face results into cells
 // with atomics; with one thread the additions land in a fixed order: internal
@@ -33,7 +36,7 @@
 // into 4 x 8 x 8 bricks of the mesh lattice (one brick per flux tile) when the
 // mesh is a lattice; results go back to the caller in the caller's order.
 //   cells   vol[nc], xc[3][nc], U[2][5][nc] (U_n / U_acc ping-pong), R[5][nc],
-//           V[5][nc], grad[15][nc], phi[5][nc], slot[6][nc] (int32)
+//           V[2][5][nc] (stage primitives, even/odd stages), grad[15][nc], phi[5][nc], slot[6][nc] (int32)
 //   faces   n[3][nf], frame[10][nf] (|n|, unit n, t, b: geometry-only part
 //           of the Roe flux, computed once at setup), xf[3][nf], left[nf],
 //           right[nf], kind[nf] (int8), wall[nf]; internal faces stored in
@@ -527,11 +530,16 @@ __global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const T* __restrict__ 
 // internal face (index = LDS slot), 4 + kind - 1 = boundary face of that kind
 // (index = boundary-face number).
 template <bool kSecondOrder, class Visc, class T>
+// Un, Vnext, alpha_next: with Vnext set (stages 0-2) the cell's primitives
+// for the next stage, primitives(Un + alpha_next R) -- update<Device> and
+// ComputePrimitives of that stage, the same arithmetic as k_prim -- are
+// written to Vnext and R is not stored; on the last stage (Vnext null) R is.
 __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, int tile, T* lds,
                                           const T* __restrict__ V, const T* __restrict__ grad,
                                           const T* __restrict__ phi, const T* __restrict__ bc,
                                           const double* Uacc_in, double* Uacc_out, double* __restrict__ R,
-                                          const double* __restrict__ dt_dev, double beta, const Visc& visc) {
+                                          const double* __restrict__ dt_dev, double beta, const Visc& visc,
+                                          const double* __restrict__ Un, T* __restrict__ Vnext, double alpha_next) {
   const int f0 = t.ptr[tile];
   const int nfl = t.ptr[tile + 1] - f0;
   const int mf = t.max_faces;
@@ -581,12 +589,24 @@ __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, in
   }
   const double dt = dt_dev[t.tcase[tile]];
   const double vol = m.vol[c];
+  double rr[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
     double r = 0.0;
     r = r + dt / vol * acc[k];
-    R[(size_t)k * nc + c] = r;
+    rr[k] = r;
     Uacc_out[(size_t)k * nc + c] = Uacc_in[(size_t)k * nc + c] + beta * r;
+  }
+  if (Vnext) {
+    double u[5], v[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) u[k] = Un[(size_t)k * nc + c] + alpha_next * rr[k];
+    ma::primitives(u, v);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Vnext[(size_t)k * nc + c] = T(v[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) R[(size_t)k * nc + c] = rr[k];
   }
 }
 
@@ -598,7 +618,8 @@ __global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const T* _
                                                 const double* Uacc_in,  // may alias Uacc_out
                                                 double* Uacc_out, double* __restrict__ R,
                                                 const double* __restrict__ dt_dev, double beta,
-                                                Visc visc) {
+                                                Visc visc, const double* __restrict__ Un, T* Vnext,
+                                                double alpha_next) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* lds = reinterpret_cast<T*>(lds_raw);  // 5 planes of t.max_faces
   // XCD-aware order: the workgroups an XCD receives (every kXcds-th) take a
@@ -611,7 +632,7 @@ __global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const T* _
     const int tile = xcd * t.per_xcd + j;
     if (tile >= t.ntiles) break;  // uniform over the workgroup
     flux_tile<kSecondOrder, Visc, T>(m, t, tile, lds, V, grad, phi, bc, Uacc_in, Uacc_out, R, dt_dev, beta,
-                                     visc);
+                                     visc, Un, Vnext, alpha_next);
     __syncthreads();  // the LDS is reused by the next tile
   }
 }
@@ -772,7 +793,7 @@ struct CaseView {
   TileMap t;
   double* U[2];
   double* R;
-  void *V, *grad, *phi, *bc;  // arithmetic type of the launch
+  void *V[2], *grad, *phi, *bc;  // arithmetic type of the launch; V[stage & 1] holds the stage's primitives
   double *dt, *time, *partial;
   const double* cfl;
   const int* case_tiles;  // {0, ntiles}
@@ -792,6 +813,7 @@ __device__ __forceinline__ int find_case(const int* __restrict__ start, int n, i
   return lo;
 }
 
+// Stage 0 only (the flux kernel of stages 0-2 writes the next stage's primitives).
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_prim_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
                                                        int n, double alpha, int parity) {
@@ -799,37 +821,37 @@ __global__ __launch_bounds__(kBlock) void k_prim_multi(const CaseView* __restric
   const int k = find_case(start, n, g);
   const CaseView& v = cv[k];
   const int c = (g - start[k]) * kBlock + threadIdx.x;
-  if (c < v.m.nc) prim_cell(v.m.nc, v.U[v.parity ^ parity], v.R, alpha, static_cast<T*>(v.V), c);
+  if (c < v.m.nc) prim_cell(v.m.nc, v.U[v.parity ^ parity], v.R, alpha, static_cast<T*>(v.V[0]), c);
 }
 
 template <bool kSecondOrder, class Limiter, class T>
 __global__ __launch_bounds__(kBlock) void k_grad_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
-                                                       int n, int per_xcd) {
+                                                       int n, int per_xcd, int vsel) {
   const int g = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
   if (g >= start[n]) return;
   const int k = find_case(start, n, g);
   const CaseView& v = cv[k];
   const int c = (g - start[k]) * kBlock + threadIdx.x;
   if (c < v.m.nc)
-    grad_cell<kSecondOrder, Limiter, T>(v.m, static_cast<const T*>(v.V), static_cast<T*>(v.grad),
+    grad_cell<kSecondOrder, Limiter, T>(v.m, static_cast<const T*>(v.V[vsel]), static_cast<T*>(v.grad),
                                         static_cast<T*>(v.phi), c);
 }
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_bc_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
-                                                     int n, double prandtl) {
+                                                     int n, double prandtl, int vsel) {
   const int g = blockIdx.x;
   const int k = find_case(start, n, g);
   const CaseView& v = cv[k];
   const int i = (g - start[k]) * kBlock + threadIdx.x;
   if (i < v.m.nf - v.m.nif)
-    bc_face(v.m, static_cast<const T*>(v.V), prandtl, v.inflow, v.bc_case, static_cast<T*>(v.bc), i);
+    bc_face(v.m, static_cast<const T*>(v.V[vsel]), prandtl, v.inflow, v.bc_case, static_cast<T*>(v.bc), i);
 }
 
 template <bool kSecondOrder, class Visc, class T>
 __global__ __launch_bounds__(kTile) void k_flux_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
-                                                      int n, int per_xcd, int parity, int first_stage, double beta,
-                                                      Visc visc) {
+                                                      int n, int per_xcd, int parity, int irk, double beta,
+                                                      double alpha_next, Visc visc) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int g = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
   if (g >= start[n]) return;  // whole workgroup, before any barrier
@@ -838,9 +860,10 @@ __global__ __launch_bounds__(kTile) void k_flux_multi(const CaseView* __restrict
   const int cur = v.parity ^ parity;
   double* Uacc = v.U[1 - cur];
   flux_tile<kSecondOrder, Visc, T>(v.m, v.t, g - start[k], reinterpret_cast<T*>(lds_raw),
-                                   static_cast<const T*>(v.V), static_cast<const T*>(v.grad),
+                                   static_cast<const T*>(v.V[irk & 1]), static_cast<const T*>(v.grad),
                                    static_cast<const T*>(v.phi), static_cast<const T*>(v.bc),
-                                   first_stage ? v.U[cur] : Uacc, Uacc, v.R, v.dt, beta, visc);
+                                   irk == 0 ? v.U[cur] : Uacc, Uacc, v.R, v.dt, beta, visc, v.U[cur],
+                                   irk < 3 ? static_cast<T*>(v.V[1 - (irk & 1)]) : nullptr, alpha_next);
 }
 
 __global__ __launch_bounds__(kTile) void k_dt_partial_multi(const CaseView* __restrict__ cv,
@@ -904,7 +927,7 @@ struct ma_solver {
   double *U[2] = {nullptr, nullptr}, *R = nullptr;
   // Arithmetic-type buffers (double, or float for the FP32 form): primitives,
   // gradients, limiters, boundary contributions.
-  void *V = nullptr, *grad = nullptr, *phi = nullptr, *bc = nullptr;
+  void *V = nullptr, *V2 = nullptr, *grad = nullptr, *phi = nullptr, *bc = nullptr;
   bool fp32 = false;
   // Cases (independent meshes stepped together; a plain run is one case).
   int ncases = 1;
@@ -1747,7 +1770,8 @@ ma_status build_planned(ma_solver* s, RunPlan& P) {
   const size_t n5 = (size_t)5 * nc;
   const size_t tsize = s->fp32 ? sizeof(float) : sizeof(double);
   if ((st = s->alloc(&s->U[0], n5)) || (st = s->alloc(&s->U[1], n5)) || (st = s->alloc(&s->R, n5)) ||
-      (st = s->alloc_bytes(&s->V, n5 * tsize)) || (st = s->alloc_bytes(&s->grad, 3 * n5 * tsize)) ||
+      (st = s->alloc_bytes(&s->V, n5 * tsize)) || (st = s->alloc_bytes(&s->V2, n5 * tsize)) ||
+      (st = s->alloc_bytes(&s->grad, 3 * n5 * tsize)) ||
       (st = s->alloc_bytes(&s->phi, n5 * tsize)) ||
       (st = s->alloc_bytes(&s->bc, (size_t)5 * std::max<long long>(nbf, 1) * tsize)) ||
       (st = s->alloc(&s->partial, (size_t)std::max(1, s->t.ntiles))) || (st = s->alloc(&s->counter, 1)) ||
@@ -1784,16 +1808,20 @@ void run_stage(ma_solver* s, int irk) {
   const int nb = nblocks(m.nc);
   double* Un = s->U[s->cur];
   double* Uacc = s->U[1 - s->cur];
-  s->launch("k_prim", [&] {
-    hipLaunchKernelGGL(k_prim<T>, dim3(nb), dim3(kBlock), 0, s->stream, m.nc, Un, s->R, alpha[irk],
-                       static_cast<T*>(s->V));
-  });
+  // the stage's primitives: V for stages 0 and 2, V2 for 1 and 3; stage 0's
+  // come from k_prim, the others from the previous stage's flux kernel
+  T* Vs = static_cast<T*>((irk & 1) ? s->V2 : s->V);
+  T* Vn = irk < 3 ? static_cast<T*>((irk & 1) ? s->V : s->V2) : nullptr;
+  if (irk == 0) {
+    s->launch("k_prim", [&] {
+      hipLaunchKernelGGL(k_prim<T>, dim3(nb), dim3(kBlock), 0, s->stream, m.nc, Un, s->R, alpha[0], Vs);
+    });
+  }
   if (SO || Visc::isViscous) {
     s->launch(SO ? "k_grad_limiter" : "k_grad", [&] {
       const int per = (nb + kXcds - 1) / kXcds;
       hipLaunchKernelGGL((k_grad<SO, Lim, T>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, m,
-                         static_cast<const T*>(s->V), static_cast<T*>(s->grad), static_cast<T*>(s->phi),
-                         per);
+                         static_cast<const T*>(Vs), static_cast<T*>(s->grad), static_cast<T*>(s->phi), per);
     });
   }
   Visc visc{effective_prandtl(s->opt)};
@@ -1803,16 +1831,16 @@ void run_stage(ma_solver* s, int irk) {
   if (nbf > 0) {
     s->launch("k_bc", [&] {
       hipLaunchKernelGGL(k_bc<T>, dim3(nblocks(nbf)), dim3(kBlock), 0, s->stream, m,
-                         static_cast<const T*>(s->V), visc.prandtl, s->inflow_dev, s->bc_case,
+                         static_cast<const T*>(Vs), visc.prandtl, s->inflow_dev, s->bc_case,
                          static_cast<T*>(s->bc));
     });
   }
   const size_t lds = (size_t)s->t.max_faces * 5 * sizeof(T);
   s->launch("k_flux", [&] {
     hipLaunchKernelGGL((k_flux<SO, Visc, T>), dim3(resident(s, s->t.per_xcd) * kXcds), dim3(kTile), lds, s->stream, m,
-                       s->t, static_cast<const T*>(s->V), static_cast<const T*>(s->grad),
+                       s->t, static_cast<const T*>(Vs), static_cast<const T*>(s->grad),
                        static_cast<const T*>(s->phi), static_cast<const T*>(s->bc), acc_in, Uacc, s->R,
-                       s->dt_dev, beta[irk], visc);
+                       s->dt_dev, beta[irk], visc, Un, Vn, irk < 3 ? alpha[irk + 1] : 0.0);
   });
 }
 
@@ -2549,7 +2577,8 @@ ma_status multi_set(Multi* M, ma_solver* const* cases, const int* parity, int n)
     v.U[0] = s->U[0];
     v.U[1] = s->U[1];
     v.R = s->R;
-    v.V = s->V;
+    v.V[0] = s->V;
+    v.V[1] = s->V2;
     v.grad = s->grad;
     v.phi = s->phi;
     v.bc = s->bc;
@@ -2601,17 +2630,18 @@ void multi_step_t(ma::Multi* M, int parity) {
   const int per_cells = (M->totals[0] + kXcds - 1) / kXcds;
   const int per_tiles = (M->totals[2] + kXcds - 1) / kXcds;
   const size_t lds = (size_t)M->max_faces * 5 * sizeof(T);
+  hipLaunchKernelGGL(k_prim_multi<T>, dim3(M->totals[0]), dim3(kBlock), 0, M->stream, M->d_views, sc, n,
+                     alpha[0], parity);
   for (int irk = 0; irk < 4; ++irk) {
-    hipLaunchKernelGGL(k_prim_multi<T>, dim3(M->totals[0]), dim3(kBlock), 0, M->stream, M->d_views, sc, n,
-                       alpha[irk], parity);
     if (SO || Visc::isViscous)
       hipLaunchKernelGGL((k_grad_multi<SO, Lim, T>), dim3(per_cells * kXcds), dim3(kBlock), 0, M->stream,
-                         M->d_views, sc, n, per_cells);
+                         M->d_views, sc, n, per_cells, irk & 1);
     if (M->totals[1] > 0)
       hipLaunchKernelGGL(k_bc_multi<T>, dim3(M->totals[1]), dim3(kBlock), 0, M->stream, M->d_views, sb, n,
-                         prandtl);
+                         prandtl, irk & 1);
     hipLaunchKernelGGL((k_flux_multi<SO, Visc, T>), dim3(per_tiles * kXcds), dim3(kTile), lds, M->stream,
-                       M->d_views, stl, n, per_tiles, parity, irk == 0 ? 1 : 0, beta[irk], visc);
+                       M->d_views, stl, n, per_tiles, parity, irk, beta[irk], irk < 3 ? alpha[irk + 1] : 0.0,
+                       visc);
   }
 }
 
