@@ -264,16 +264,27 @@ __global__ __launch_bounds__(kBlock) void k_prim(int nc, const double* __restric
 // Green-Gauss gradient (GreenGauss.h:51-270), then, for second order, the
 // stencil min/max (StencilLimiter.h:56-282) and the limiter
 // (StencilLimiter.h:288-500) of one cell, all from the cell's slot list.
+// nbr: kMaxSlots planes of nc, the cell across each slot's internal face (the
+// cell itself for a boundary face or an empty slot).  A boundary face (side
+// 0, neighbour = the cell) goes through the internal-face formulas with the
+// same bits as the reference's boundary terms -- 0.5 * (v + v) == v and
+// min(v, v) == v exactly -- so the slot loop has no branch; all six slots'
+// area vectors and neighbour values are loaded before the first is used (one
+// round of gathers per wavefront instead of one per face, measured 8-16%
+// faster than the per-face form), and empty slots are masked.
 template <bool kSecondOrder, class Limiter, class T>
-__device__ __forceinline__ void grad_cell(const DevMesh& m, const T* __restrict__ V, T* __restrict__ grad,
-                                          T* __restrict__ phi, int c) {
+__device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restrict__ nbr, const T* __restrict__ V,
+                                          T* __restrict__ grad, T* __restrict__ phi, int c) {
   const int nc = m.nc, nf = m.nf;
+  int ent[kMaxSlots], nb[kMaxSlots];
+#pragma unroll
+  for (int s = 0; s < kMaxSlots; ++s) {
+    ent[s] = m.slot[(size_t)s * nc + c];
+    nb[s] = nbr[(size_t)s * nc + c];
+  }
   T vc[5];
   load5(V, nc, c, vc);
   const T vol = T(m.vol[c]);
-  int ent[kMaxSlots];
-#pragma unroll
-  for (int s = 0; s < kMaxSlots; ++s) ent[s] = m.slot[(size_t)s * nc + c];
 
   T g[5][3];
 #pragma unroll
@@ -287,47 +298,37 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const T* __restrict_
     mx[k] = -T(1.0e300);
   }
 
+  T an[kMaxSlots][3], vnn[kMaxSlots][5];  // empty slots read face 0 and the cell itself
 #pragma unroll
   for (int s = 0; s < kMaxSlots; ++s) {
-    const int e = ent[s];
-    if (e < 0) break;
-    const int f = e >> 1;
-    T n[3];
-    load3(area<T>(m), nf, f, n);
-    if (f < m.nif) {
-      const int side = e & 1;
-      const int nb = side ? m.fl[f] : m.fr[f];
-      T vn[5];
-      load5(V, nc, nb, vn);
-      const T* vl = side ? vn : vc;
-      const T* vr = side ? vc : vn;
+    load3(area<T>(m), nf, ent[s] < 0 ? 0 : ent[s] >> 1, an[s]);
+    load5(V, nc, nb[s], vnn[s]);
+  }
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
+  for (int s = 0; s < kMaxSlots; ++s) {
+    const bool ok = ent[s] >= 0;
+    const int e = ok ? ent[s] : 0;
+    const int side = e & 1;
+    const T* n = an[s];
+    const T* vn = vnn[s];
 #pragma unroll
-        for (int k = 0; k < 5; ++k) {
-          const T gg = T(0.5) * (vl[k] + vr[k]) * n[d];
-          g[k][d] += side ? -gg / vol : gg / vol;
-        }
-      if (kSecondOrder) {
+    for (int d = 0; d < 3; ++d)
 #pragma unroll
-        for (int k = 0; k < 5; ++k) {
-          const T fmin = vr[k] < vl[k] ? vr[k] : vl[k];
-          const T fmax = vr[k] > vl[k] ? vr[k] : vl[k];
-          mn[k] = ma::smin(mn[k], fmin);
-          mx[k] = ma::smax(mx[k], fmax);
-        }
+      for (int k = 0; k < 5; ++k) {
+        const T vl = side ? vn[k] : vc[k];
+        const T vr = side ? vc[k] : vn[k];
+        const T gg = ok ? T(0.5) * (vl + vr) * n[d] : T(0);
+        g[k][d] += side ? -gg / vol : gg / vol;
       }
-    } else {
+    if (kSecondOrder) {
 #pragma unroll
-      for (int k = 0; k < 5; ++k)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) g[k][d] += vc[k] * n[d] / vol;
-      if (kSecondOrder) {
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-          mn[k] = ma::smin(mn[k], vc[k]);
-          mx[k] = ma::smax(mx[k], vc[k]);
-        }
+      for (int k = 0; k < 5; ++k) {
+        const T vl = side ? vn[k] : vc[k];
+        const T vr = side ? vc[k] : vn[k];
+        const T fmin = vr < vl ? vr : vl;
+        const T fmax = vr > vl ? vr : vl;
+        mn[k] = ok ? ma::smin(mn[k], fmin) : mn[k];
+        mx[k] = ok ? ma::smax(mx[k], fmax) : mx[k];
       }
     }
   }
@@ -348,14 +349,13 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const T* __restrict_
   for (int k = 0; k < 5; ++k) lim[k] = T(1.0);
 #pragma unroll
   for (int s = 0; s < kMaxSlots; ++s) {
-    const int e = ent[s];
-    if (e < 0) break;
-    const int f = e >> 1;
-    // displacement from the FP64 centroids, rounded to the arithmetic type
+    const bool ok = ent[s] >= 0;
+    const int e = ok ? ent[s] : 0;
+    // displacement x_f - x_c from the FP64 centroids, rounded to the arithmetic type
     T disp[3], dist = 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      disp[d] = static_cast<const T*>(m.fdx)[(size_t)((e & 1) * 3 + d) * nf + f];
+      disp[d] = static_cast<const T*>(m.fdx)[(size_t)((e & 1) * 3 + d) * nf + (e >> 1)];
       dist += disp[d] * disp[d];
     }
 #pragma unroll
@@ -365,7 +365,8 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const T* __restrict_
       for (int d = 0; d < 3; ++d) du += disp[d] * g[k][d];
       const T dumax = mx[k] - vc[k];
       const T dumin = mn[k] - vc[k];
-      lim[k] = ma::smin(lim[k], Limiter::limit(dumax, dumin, du, dist));
+      const T l = Limiter::limit(dumax, dumin, du, dist);
+      lim[k] = ok ? ma::smin(lim[k], l) : lim[k];
     }
   }
 #pragma unroll
@@ -373,13 +374,13 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const T* __restrict_
 }
 
 template <bool kSecondOrder, class Limiter, class T>
-__global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const T* __restrict__ V,
+__global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const int* __restrict__ nbr, const T* __restrict__ V,
                                                  T* __restrict__ grad, T* __restrict__ phi,
                                                  int per_xcd) {
   // XCD-aware: each XCD takes a contiguous run of blocks (see k_flux)
   const int b = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
   const int c = b * kBlock + threadIdx.x;
-  if (c < m.nc) grad_cell<kSecondOrder, Limiter, T>(m, V, grad, phi, c);
+  if (c < m.nc) grad_cell<kSecondOrder, Limiter, T>(m, nbr, V, grad, phi, c);
 }
 
 // Reconstructed (second order) primitive state of cell `c` at face centroid xf:
@@ -799,6 +800,7 @@ struct CaseView {
   const int* case_tiles;  // {0, ntiles}
   const Inflow* inflow;
   const int* bc_case;
+  const int* nbr;  // neighbour-cell planes (grad_cell)
   int parity;  // U[parity ^ step parity] holds the case's sol_n
 };
 
@@ -833,7 +835,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_multi(const CaseView* __restric
   const CaseView& v = cv[k];
   const int c = (g - start[k]) * kBlock + threadIdx.x;
   if (c < v.m.nc)
-    grad_cell<kSecondOrder, Limiter, T>(v.m, static_cast<const T*>(v.V[vsel]), static_cast<T*>(v.grad),
+    grad_cell<kSecondOrder, Limiter, T>(v.m, v.nbr, static_cast<const T*>(v.V[vsel]), static_cast<T*>(v.grad),
                                         static_cast<T*>(v.phi), c);
 }
 
@@ -943,6 +945,7 @@ struct ma_solver {
   double *norm_partial = nullptr, *norm_out = nullptr;  // per tile x 5, per case x 5
   const double* xc = nullptr;  // cell centroids (3 planes), for the initial condition
   const int* org = nullptr;    // device cell -> caller's cell index
+  const int* nbr = nullptr;    // kMaxSlots planes of nc: the cell across each slot (grad_cell)
   int cur = 0;  // U[cur] holds sol_n
   long long steps_done = 0;
   // HIP graphs of graph_steps whole RK4 steps (an even count, so U[cur] is
@@ -1722,6 +1725,23 @@ ma_status build_planned(ma_solver* s, RunPlan& P) {
   m.fr = dfr;
   m.fkind = dkind;
   m.fwall = dfw;
+  // neighbour planes: the cell across each slot's internal face, else the cell itself
+  {
+    int* dnbr = nullptr;
+    if (s->counting) {
+      st = s->alloc(&dnbr, (size_t)kMaxSlots * nc);
+    } else {
+      std::vector<int> h((size_t)kMaxSlots * nc);
+      for (int q = 0; q < kMaxSlots; ++q)
+        for (int c = 0; c < nc; ++c) {
+          const int e = L.slot[(size_t)q * nc + c], f = e >> 1;
+          h[(size_t)q * nc + c] = (e >= 0 && f < nif) ? ((e & 1) ? L.fl[f] : L.fr[f]) : c;
+        }
+      st = upload(s, &dnbr, h);
+    }
+    if (st) return st;
+    s->nbr = dnbr;
+  }
 
   int *dtptr, *dtfaces, *dtslot, *dtcptr, *dtcase;
   if ((st = upload(s, &dtptr, L.tptr)) || (st = upload(s, &dtfaces, L.tfaces)) || (st = upload(s, &dtslot, L.tslot)) ||
@@ -1820,7 +1840,7 @@ void run_stage(ma_solver* s, int irk) {
   if (SO || Visc::isViscous) {
     s->launch(SO ? "k_grad_limiter" : "k_grad", [&] {
       const int per = (nb + kXcds - 1) / kXcds;
-      hipLaunchKernelGGL((k_grad<SO, Lim, T>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, m,
+      hipLaunchKernelGGL((k_grad<SO, Lim, T>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, m, s->nbr,
                          static_cast<const T*>(Vs), static_cast<T*>(s->grad), static_cast<T*>(s->phi), per);
     });
   }
@@ -2589,6 +2609,7 @@ ma_status multi_set(Multi* M, ma_solver* const* cases, const int* parity, int n)
     v.case_tiles = s->case_tiles;
     v.inflow = s->inflow_dev;
     v.bc_case = s->bc_case;
+    v.nbr = s->nbr;
     v.parity = parity[k];
     st[k + 1] = st[k] + nblocks(s->m.nc);
     st[cap1 + k + 1] = st[cap1 + k] + nblocks(s->m.nf - s->m.nif);
