@@ -35,12 +35,20 @@
 // wavefront's 64 lanes read 64 consecutive doubles).  Cells are renumbered
 // into 4 x 8 x 8 bricks of the mesh lattice (one brick per flux tile) when the
 // mesh is a lattice; results go back to the caller in the caller's order.
-//   cells   vol[nc], xc[3][nc], U[2][5][nc] (U_n / U_acc ping-pong), R[5][nc],
-//           V[2][5][nc] (stage primitives, even/odd stages), grad[15][nc], phi[5][nc], slot[6][nc] (int32)
-//   faces   n[3][nf], frame[10][nf] (|n|, unit n, t, b: geometry-only part
-//           of the Roe flux, computed once at setup), xf[3][nf], left[nf],
-//           right[nf], kind[nf] (int8), wall[nf]; internal faces stored in
-//           processing (left-cell) order, then the boundary groups
+//   cells   vol[nc], U[2][5][nc] (U_n / U_acc ping-pong), R[5][nc],
+//           V[2][5][nc] (stage primitives, even/odd stages), grad[15][nc],
+//           phi[5][nc]; int32 slot[6][nc] (face*2 + side, reference order),
+//           nbr[6][nc] (the cell across each slot), tile slot codes [6][nc];
+//           xc[3][nc] for the initial condition only
+//   faces   n[3][nf] (area vector; an FP32 copy in the FP32 form),
+//           disp[6][nf] (x_f - x_c of the left and right cell, arithmetic
+//           type), left[nf], right[nf], kind[nf] (int8), wall[nf]; the unit
+//           frame (|n|, n, t, b) is rebuilt from n in the kernels, and stored
+//           as frame[10][nf] only for meshes whose tangents do not follow the
+//           reference construction; internal faces in processing (left-cell)
+//           order, then the boundary groups
+//   tiles   ptr[ntiles+1], faces[] (internal faces touching the tile) with
+//           lr[] (their left and right cells), cptr[ntiles+1], case[ntiles]
 // Entry point declarations and the reference interfaces they replace are in
 // include/miniaero.h.
 #include <hip/hip_runtime.h>
@@ -100,6 +108,7 @@ struct TileMap {
   int max_faces;                  // largest ptr[b+1] - ptr[b]
   const int* __restrict__ ptr;    // ntiles + 1
   const int* __restrict__ faces;  // device face indices
+  const int2* __restrict__ lr;    // parallel to faces: the face's left and right cells
   const int* __restrict__ slot;   // kMaxSlots planes of nc, see k_flux
   const int* __restrict__ cptr;   // ntiles + 1: cells of tile b are [cptr[b], cptr[b+1])
   const int* __restrict__ tcase;  // ntiles: the case (independent mesh) tile b belongs to
@@ -129,12 +138,12 @@ __device__ __forceinline__ const float* area<float>(const DevMesh& m) {
   return m.fn32;
 }
 
+// Face frame of face f from its area vector a (already loaded): rebuilt from
+// a, or read from the stored frame planes.
 template <class T>
-__device__ __forceinline__ ma::FaceFrameT<T> load_frame(const DevMesh& m, int f) {
+__device__ __forceinline__ ma::FaceFrameT<T> frame_of(const DevMesh& m, int f, const T* a) {
   ma::FaceFrameT<T> fr;
   if (m.derive_frame) {
-    T a[3];
-    load3(area<T>(m), m.nf, f, a);
     const ma::FaceFrameT<T> d = ma::frame_from_area_t<T>(a);
     fr.nn = d.nn;
 #pragma unroll
@@ -146,7 +155,8 @@ __device__ __forceinline__ ma::FaceFrameT<T> load_frame(const DevMesh& m, int f)
     }
     return fr;
   }
-  load3(area<T>(m), m.nf, f, fr.n);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) fr.n[d] = a[d];
   const double* p = m.ffr;
   const size_t nf = m.nf;
   fr.nn = T(p[f]);
@@ -157,6 +167,13 @@ __device__ __forceinline__ ma::FaceFrameT<T> load_frame(const DevMesh& m, int f)
     fr.bu[d] = T(p[(7 + d) * nf + f]);
   }
   return fr;
+}
+
+template <class T>
+__device__ __forceinline__ ma::FaceFrameT<T> load_frame(const DevMesh& m, int f) {
+  T a[3];
+  load3(area<T>(m), m.nf, f, a);
+  return frame_of<T>(m, f, a);
 }
 
 // ---------------------------------------------------------------- kernels --
@@ -383,48 +400,56 @@ __global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const int* __restric
   if (c < m.nc) grad_cell<kSecondOrder, Limiter, T>(m, nbr, V, grad, phi, c);
 }
 
-// Reconstructed (second order) primitive state of cell `c` at face centroid xf:
-// compute_face_flux's extrapolation, Flux.h:109-132.  The displacement is
-// taken from the FP64 coordinates, then carried in the arithmetic type T.
-// Second-order extrapolation of cell c's primitives to face f (Flux.h:107-130):
-// v += phi * (x_f - x_c) . grad, with x_f - x_c from the displacement planes
-// (side 0: the face's left cell, 1: its right cell).
-template <class T>
-__device__ __forceinline__ void reconstruct(const DevMesh& m, const T* __restrict__ grad,
-                                            const T* __restrict__ phi, int c, int f, int side, T* v) {
-  const int nc = m.nc;
-  const T* dxp = static_cast<const T*>(m.fdx) + (size_t)side * 3 * m.nf + f;
-  T dx[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) dx[d] = dxp[(size_t)d * m.nf];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    T t = 0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) t += dx[d] * grad[(size_t)(k * 3 + d) * nc + c];
-    v[k] += t * phi[(size_t)k * nc + c];
-  }
-}
-
-// Flux of internal face f: compute_face_flux's body (Flux.h:89-160) without
-// the scatter -- primitives of both cells, second-order extrapolation with
-// the cell limiters, Roe, minus the viscous flux from face-averaged
-// gradients.  The result is a pure function of (f, V, grad, phi), so every
-// workgroup that evaluates face f gets the same bits.
+// Flux of internal face f between cells L and Rr: compute_face_flux's body
+// (Flux.h:89-160) without the scatter -- primitives of both cells, the
+// second-order extrapolation v += phi * ((x_f - x_c) . grad) with the cell
+// limiters (Flux.h:107-130; x_f - x_c from the displacement planes, side 0
+// the left cell, 1 the right), Roe, minus the viscous flux from
+// face-averaged gradients.  Every input is loaded before the first is used.
+// The result is a pure function of (f, V, grad, phi), so every workgroup
+// that evaluates face f gets the same bits.
 template <bool kSecondOrder, class Visc, class T>
 __device__ __forceinline__ void interior_flux(const DevMesh& m, const T* __restrict__ V,
                                               const T* __restrict__ grad,
-                                              const T* __restrict__ phi, int f,
+                                              const T* __restrict__ phi, int f, int L, int Rr,
                                               const Visc& visc, T* F) {
-  const int nc = m.nc;
-  const int L = m.fl[f], Rr = m.fr[f];
-  T vl[5], vr[5];
+  constexpr bool kGrad = kSecondOrder || Visc::isViscous;
+  const int nc = m.nc, nf = m.nf;
+  T vl[5], vr[5], a[3];
+  T gl[kGrad ? 15 : 1], gr[kGrad ? 15 : 1], pl[5], pr[5], dl[3], dr[3];
   load5(V, nc, L, vl);
   load5(V, nc, Rr, vr);
-  const ma::FaceFrameT<T> fr = load_frame<T>(m, f);
+  load3(area<T>(m), nf, f, a);
+  if (kGrad) {
+#pragma unroll
+    for (int j = 0; j < 15; ++j) {
+      gl[j] = grad[(size_t)j * nc + L];
+      gr[j] = grad[(size_t)j * nc + Rr];
+    }
+  }
   if (kSecondOrder) {
-    reconstruct(m, grad, phi, L, f, 0, vl);
-    reconstruct(m, grad, phi, Rr, f, 1, vr);
+    load5(phi, nc, L, pl);
+    load5(phi, nc, Rr, pr);
+    const T* dx = static_cast<const T*>(m.fdx);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      dl[d] = dx[(size_t)d * nf + f];
+      dr[d] = dx[(size_t)(3 + d) * nf + f];
+    }
+  }
+  const ma::FaceFrameT<T> fr = frame_of<T>(m, f, a);
+  if (kSecondOrder) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      T tl = 0, tr = 0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        tl += dl[d] * gl[k * 3 + d];
+        tr += dr[d] * gr[k * 3 + d];
+      }
+      vl[k] += tl * pl[k];
+      vr[k] += tr * pr[k];
+    }
   }
   const ma::RoeFlux roe;
   roe.flux_frame(vl, vr, F, fr);
@@ -434,8 +459,7 @@ __device__ __forceinline__ void interior_flux(const DevMesh& m, const T* __restr
     for (int k = 0; k < 5; ++k) {
       vf[k] = T(0.5) * (vl[k] + vr[k]);
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
-        gf[k][d] = T(0.5) * (grad[(size_t)(k * 3 + d) * nc + L] + grad[(size_t)(k * 3 + d) * nc + Rr]);
+      for (int d = 0; d < 3; ++d) gf[k][d] = T(0.5) * (gl[k * 3 + d] + gr[k * 3 + d]);
     }
     visc.compute_flux(gf, vf, fr.n, fv);
 #pragma unroll
@@ -544,9 +568,22 @@ __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, in
   const int f0 = t.ptr[tile];
   const int nfl = t.ptr[tile + 1] - f0;
   const int mf = t.max_faces;
+  // the next face's indices are fetched while the current face is evaluated
+  int f = 0;
+  int2 lr = make_int2(0, 0);
+  if (threadIdx.x < nfl) {
+    f = t.faces[f0 + threadIdx.x];
+    lr = t.lr[f0 + threadIdx.x];
+  }
   for (int i = threadIdx.x; i < nfl; i += kTile) {
+    const int fc = f;
+    const int2 lrc = lr;
+    if (i + kTile < nfl) {
+      f = t.faces[f0 + i + kTile];
+      lr = t.lr[f0 + i + kTile];
+    }
     T F[5];
-    interior_flux<kSecondOrder, Visc>(m, V, grad, phi, t.faces[f0 + i], visc, F);
+    interior_flux<kSecondOrder, Visc>(m, V, grad, phi, fc, lrc.x, lrc.y, visc, F);
 #pragma unroll
     for (int k = 0; k < 5; ++k) lds[k * mf + i] = F[k];
   }
@@ -611,8 +648,14 @@ __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, in
   }
 }
 
+// Waves per SIMD the flux kernels are register-budgeted for: FP64 3 (the
+// compiler's own choice lands one register over the 168 that allow 3, at 2
+// waves, measured 10% slower); FP32 no constraint.
+template <class T>
+constexpr int kFluxWaves = sizeof(T) == 8 ? 3 : 1;
+
 template <bool kSecondOrder, class Visc, class T>
-__global__ __launch_bounds__(kTile) void k_flux(DevMesh m, TileMap t, const T* __restrict__ V,
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kFluxWaves<T>))) void k_flux(DevMesh m, TileMap t, const T* __restrict__ V,
                                                 const T* __restrict__ grad,
                                                 const T* __restrict__ phi,
                                                 const T* __restrict__ bc,
@@ -851,7 +894,7 @@ __global__ __launch_bounds__(kBlock) void k_bc_multi(const CaseView* __restrict_
 }
 
 template <bool kSecondOrder, class Visc, class T>
-__global__ __launch_bounds__(kTile) void k_flux_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kFluxWaves<T>))) void k_flux_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
                                                       int n, int per_xcd, int parity, int irk, double beta,
                                                       double alpha_next, Visc visc) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -1752,6 +1795,19 @@ ma_status build_planned(ma_solver* s, RunPlan& P) {
   s->t.max_faces = L.max_faces;
   s->t.ptr = dtptr;
   s->t.faces = dtfaces;
+  {
+    int2* dlr = nullptr;
+    const size_t ntf = L.tfaces.size();
+    if (s->counting) {
+      st = s->alloc(&dlr, ntf);
+    } else {
+      std::vector<int2> h(ntf);
+      for (size_t i = 0; i < ntf; ++i) h[i] = make_int2(L.fl[L.tfaces[i]], L.fr[L.tfaces[i]]);
+      st = upload(s, &dlr, h);
+    }
+    if (st) return st;
+    s->t.lr = dlr;
+  }
   s->t.slot = dtslot;
   s->t.cptr = dtcptr;
   s->t.tcase = dtcase;
