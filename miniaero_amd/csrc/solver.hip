@@ -592,6 +592,15 @@ __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, in
   if (c >= t.cptr[tile + 1]) return;  // (the caller's barrier is reached by every thread)
   const int nc = m.nc;
   const int nbf = m.nf - m.nif;
+  // The cell's accumulator and base state are loaded before anything is
+  // stored: Uacc_in is Uacc_out or a different array, never a shifted view,
+  // so reading all five planes first reads the same values, and the loads
+  // need not wait behind the stores.
+  double uin[5], un[5];
+  load5(Uacc_in, nc, c, uin);
+  if (Vnext) load5(Un, nc, c, un);
+  const double dt = dt_dev[t.tcase[tile]];
+  const double vol = m.vol[c];
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   int ent[kMaxSlots];
 #pragma unroll
@@ -625,20 +634,18 @@ __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, in
         for (int k = 0; k < 5; ++k) acc[k] += double(bc[(size_t)k * nbf + j]);
       }
   }
-  const double dt = dt_dev[t.tcase[tile]];
-  const double vol = m.vol[c];
   double rr[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
     double r = 0.0;
     r = r + dt / vol * acc[k];
     rr[k] = r;
-    Uacc_out[(size_t)k * nc + c] = Uacc_in[(size_t)k * nc + c] + beta * r;
+    Uacc_out[(size_t)k * nc + c] = uin[k] + beta * r;
   }
   if (Vnext) {
     double u[5], v[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) u[k] = Un[(size_t)k * nc + c] + alpha_next * rr[k];
+    for (int k = 0; k < 5; ++k) u[k] = un[k] + alpha_next * rr[k];
     ma::primitives(u, v);
 #pragma unroll
     for (int k = 0; k < 5; ++k) Vnext[(size_t)k * nc + c] = T(v[k]);
@@ -699,13 +706,18 @@ __device__ __forceinline__ void dt_tile(const DevMesh& m, const TileMap& t, int 
     load5(Un, nc, c, u);
     ma::primitives(u, v);
     const double snd = sqrt(1.4 * (v[0] * ma::kRgas * v[4]) / v[0]);
+    // every slot's area vector is loaded before the sums (empty slots read face 0)
+    int ent[kMaxSlots];
+    double an[kMaxSlots][3];
+#pragma unroll
+    for (int q = 0; q < kMaxSlots; ++q) ent[q] = m.slot[(size_t)q * nc + c];
+#pragma unroll
+    for (int q = 0; q < kMaxSlots; ++q) load3(m.fn, nf, ent[q] < 0 ? 0 : ent[q] >> 1, an[q]);
     double lc = 0.0, lv = 0.0;
+#pragma unroll
     for (int q = 0; q < kMaxSlots; ++q) {
-      const int e = m.slot[(size_t)q * nc + c];
-      if (e < 0) break;
-      const int f = e >> 1;
-      double n[3];
-      load3(m.fn, nf, f, n);
+      if (ent[q] < 0) break;
+      const double* n = an[q];
       const double n2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
       const double nn = sqrt(n2);
       lc = lc + (fabs(v[1] * n[0] + v[2] * n[1] + v[3] * n[2]) + snd * nn);
