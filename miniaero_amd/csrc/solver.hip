@@ -75,6 +75,13 @@ constexpr int kMaxSlots = 6;  // hex8 cells (ElementTopoHexa8); checked at setup
 constexpr int kBlock = 256;
 constexpr int kTile = 256;    // cells per workgroup of the flux kernel
 constexpr int kXcds = 8;      // MI355X dies; workgroups are dealt to them round-robin
+// Tile-face cell entries of the face-value scheme (inviscid second order):
+// cell index in the low kCellBits bits (a run has at most 2^28 cells), the
+// face's slot in that cell's slot list above them; other schemes store the
+// plain cell index.
+constexpr int kCellBits = 28;
+__device__ __forceinline__ int cell_of(int x) { return x & ((1 << kCellBits) - 1); }
+__device__ __forceinline__ int slot_of(int x) { return x >> kCellBits; }
 
 struct DevMesh {
   int nc;    // owned cells
@@ -108,7 +115,7 @@ struct TileMap {
   int max_faces;                  // largest ptr[b+1] - ptr[b]
   const int* __restrict__ ptr;    // ntiles + 1
   const int* __restrict__ faces;  // device face indices
-  const int2* __restrict__ lr;    // parallel to faces: the face's left and right cells
+  const int2* __restrict__ lr;    // parallel to faces: the face's left and right cells (+ slots, see kCellBits)
   const int* __restrict__ slot;   // kMaxSlots planes of nc, see k_flux
   const int* __restrict__ cptr;   // ntiles + 1: cells of tile b are [cptr[b], cptr[b+1])
   const int* __restrict__ tcase;  // ntiles: the case (independent mesh) tile b belongs to
@@ -289,9 +296,14 @@ __global__ __launch_bounds__(kBlock) void k_prim(int nc, const double* __restric
 // area vectors and neighbour values are loaded before the first is used (one
 // round of gathers per wavefront instead of one per face, measured 8-16%
 // faster than the per-face form), and empty slots are masked.
-template <bool kSecondOrder, class Limiter, class T>
+// kFaceVals (inviscid second order): each slot's reconstructed value -- the
+// extrapolation v + phi ((x_f - x_c) . grad) of Flux.h:107-130, the same
+// expression in the same order as the flux kernel's otherwise -- is stored
+// in fv[kMaxSlots * 5][nc], so the flux kernel gathers 5 values per face
+// side instead of 25; grad and phi are then stored only when non-null.
+template <bool kSecondOrder, bool kFaceVals, class Limiter, class T>
 __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restrict__ nbr, const T* __restrict__ V,
-                                          T* __restrict__ grad, T* __restrict__ phi, int c) {
+                                          T* __restrict__ grad, T* __restrict__ phi, T* __restrict__ fv, int c) {
   const int nc = m.nc, nf = m.nf;
   int ent[kMaxSlots], nb[kMaxSlots];
 #pragma unroll
@@ -357,11 +369,12 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
       T x = 0;
       x += g[k][d];
       g[k][d] = x;
-      grad[(size_t)(k * 3 + d) * nc + c] = x;
+      if (!kFaceVals || grad) grad[(size_t)(k * 3 + d) * nc + c] = x;
     }
   if (!kSecondOrder) return;
 
   T lim[5];
+  T dus[kFaceVals ? kMaxSlots : 1][5];  // the extrapolation increments, kept for the face values
 #pragma unroll
   for (int k = 0; k < 5; ++k) lim[k] = T(1.0);
 #pragma unroll
@@ -380,24 +393,36 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
       T du = 0;
 #pragma unroll
       for (int d = 0; d < 3; ++d) du += disp[d] * g[k][d];
+      if (kFaceVals) dus[kFaceVals ? s : 0][k] = du;
       const T dumax = mx[k] - vc[k];
       const T dumin = mn[k] - vc[k];
       const T l = Limiter::limit(dumax, dumin, du, dist);
       lim[k] = ok ? ma::smin(lim[k], l) : lim[k];
     }
   }
+  if (!kFaceVals || phi) {
 #pragma unroll
-  for (int k = 0; k < 5; ++k) phi[(size_t)k * nc + c] = lim[k];
+    for (int k = 0; k < 5; ++k) phi[(size_t)k * nc + c] = lim[k];
+  }
+  if (kFaceVals) {
+#pragma unroll
+    for (int s = 0; s < kMaxSlots; ++s) {
+      const int e = ent[s];
+      if (e < 0) break;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) fv[(size_t)(s * 5 + k) * nc + c] = vc[k] + dus[kFaceVals ? s : 0][k] * lim[k];
+    }
+  }
 }
 
-template <bool kSecondOrder, class Limiter, class T>
+template <bool kSecondOrder, bool kFaceVals, class Limiter, class T>
 __global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const int* __restrict__ nbr, const T* __restrict__ V,
-                                                 T* __restrict__ grad, T* __restrict__ phi,
+                                                 T* __restrict__ grad, T* __restrict__ phi, T* __restrict__ fv,
                                                  int per_xcd) {
   // XCD-aware: each XCD takes a contiguous run of blocks (see k_flux)
   const int b = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
   const int c = b * kBlock + threadIdx.x;
-  if (c < m.nc) grad_cell<kSecondOrder, Limiter, T>(m, nbr, V, grad, phi, c);
+  if (c < m.nc) grad_cell<kSecondOrder, kFaceVals, Limiter, T>(m, nbr, V, grad, phi, fv, c);
 }
 
 // Flux of internal face f between cells L and Rr: compute_face_flux's body
@@ -406,19 +431,28 @@ __global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const int* __restric
 // limiters (Flux.h:107-130; x_f - x_c from the displacement planes, side 0
 // the left cell, 1 the right), Roe, minus the viscous flux from
 // face-averaged gradients.  Every input is loaded before the first is used.
-// The result is a pure function of (f, V, grad, phi), so every workgroup
-// that evaluates face f gets the same bits.
+// Inviscid second order takes both sides' extrapolated values from fv (slot
+// sL of cell L, slot sR of cell Rr), stored by the gradient kernel with the
+// same arithmetic.  The result is a pure function of (f, V, grad, phi), so
+// every workgroup that evaluates face f gets the same bits.
 template <bool kSecondOrder, class Visc, class T>
 __device__ __forceinline__ void interior_flux(const DevMesh& m, const T* __restrict__ V,
                                               const T* __restrict__ grad,
-                                              const T* __restrict__ phi, int f, int L, int Rr,
-                                              const Visc& visc, T* F) {
-  constexpr bool kGrad = kSecondOrder || Visc::isViscous;
+                                              const T* __restrict__ phi, const T* __restrict__ fv, int f,
+                                              int L, int sL, int Rr, int sR, const Visc& visc, T* F) {
+  constexpr bool kFaceVals = kSecondOrder && !Visc::isViscous;
+  constexpr bool kRecon = kSecondOrder && !kFaceVals;  // extrapolated here
+  constexpr bool kGrad = Visc::isViscous;
   const int nc = m.nc, nf = m.nf;
   T vl[5], vr[5], a[3];
   T gl[kGrad ? 15 : 1], gr[kGrad ? 15 : 1], pl[5], pr[5], dl[3], dr[3];
-  load5(V, nc, L, vl);
-  load5(V, nc, Rr, vr);
+  if (kFaceVals) {
+    load5(fv + (size_t)sL * 5 * nc, nc, L, vl);
+    load5(fv + (size_t)sR * 5 * nc, nc, Rr, vr);
+  } else {
+    load5(V, nc, L, vl);
+    load5(V, nc, Rr, vr);
+  }
   load3(area<T>(m), nf, f, a);
   if (kGrad) {
 #pragma unroll
@@ -427,7 +461,7 @@ __device__ __forceinline__ void interior_flux(const DevMesh& m, const T* __restr
       gr[j] = grad[(size_t)j * nc + Rr];
     }
   }
-  if (kSecondOrder) {
+  if (kRecon) {
     load5(phi, nc, L, pl);
     load5(phi, nc, Rr, pr);
     const T* dx = static_cast<const T*>(m.fdx);
@@ -438,7 +472,7 @@ __device__ __forceinline__ void interior_flux(const DevMesh& m, const T* __restr
     }
   }
   const ma::FaceFrameT<T> fr = frame_of<T>(m, f, a);
-  if (kSecondOrder) {
+  if (kRecon) {
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       T tl = 0, tr = 0;
@@ -561,7 +595,8 @@ template <bool kSecondOrder, class Visc, class T>
 // written to Vnext and R is not stored; on the last stage (Vnext null) R is.
 __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, int tile, T* lds,
                                           const T* __restrict__ V, const T* __restrict__ grad,
-                                          const T* __restrict__ phi, const T* __restrict__ bc,
+                                          const T* __restrict__ phi, const T* __restrict__ fv,
+                                          const T* __restrict__ bc,
                                           const double* Uacc_in, double* Uacc_out, double* __restrict__ R,
                                           const double* __restrict__ dt_dev, double beta, const Visc& visc,
                                           const double* __restrict__ Un, T* __restrict__ Vnext, double alpha_next) {
@@ -583,7 +618,11 @@ __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, in
       lr = t.lr[f0 + i + kTile];
     }
     T F[5];
-    interior_flux<kSecondOrder, Visc>(m, V, grad, phi, fc, lrc.x, lrc.y, visc, F);
+    // slots are packed into the cell entries only for the face-value scheme
+    constexpr bool kFaceVals = kSecondOrder && !Visc::isViscous;
+    interior_flux<kSecondOrder, Visc>(m, V, grad, phi, fv, fc, kFaceVals ? cell_of(lrc.x) : lrc.x,
+                                      slot_of(lrc.x), kFaceVals ? cell_of(lrc.y) : lrc.y, slot_of(lrc.y),
+                                      visc, F);
 #pragma unroll
     for (int k = 0; k < 5; ++k) lds[k * mf + i] = F[k];
   }
@@ -682,8 +721,9 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kFluxWave
   for (int j = blockIdx.x / kXcds; j < t.per_xcd; j += stride) {
     const int tile = xcd * t.per_xcd + j;
     if (tile >= t.ntiles) break;  // uniform over the workgroup
-    flux_tile<kSecondOrder, Visc, T>(m, t, tile, lds, V, grad, phi, bc, Uacc_in, Uacc_out, R, dt_dev, beta,
-                                     visc, Un, Vnext, alpha_next);
+    // the face-value scheme reads no gradients: its face values come in grad
+    flux_tile<kSecondOrder, Visc, T>(m, t, tile, lds, V, grad, phi, grad, bc, Uacc_in, Uacc_out, R, dt_dev,
+                                     beta, visc, Un, Vnext, alpha_next);
     __syncthreads();  // the LDS is reused by the next tile
   }
 }
@@ -850,6 +890,7 @@ struct CaseView {
   double* U[2];
   double* R;
   void *V[2], *grad, *phi, *bc;  // arithmetic type of the launch; V[stage & 1] holds the stage's primitives
+  void* fv;                      // extrapolated face values (inviscid second order)
   double *dt, *time, *partial;
   const double* cfl;
   const int* case_tiles;  // {0, ntiles}
@@ -881,17 +922,19 @@ __global__ __launch_bounds__(kBlock) void k_prim_multi(const CaseView* __restric
   if (c < v.m.nc) prim_cell(v.m.nc, v.U[v.parity ^ parity], v.R, alpha, static_cast<T*>(v.V[0]), c);
 }
 
-template <bool kSecondOrder, class Limiter, class T>
+template <bool kSecondOrder, bool kFaceVals, class Limiter, class T>
 __global__ __launch_bounds__(kBlock) void k_grad_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
-                                                       int n, int per_xcd, int vsel) {
+                                                       int n, int per_xcd, int vsel, int store_grad) {
   const int g = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
   if (g >= start[n]) return;
   const int k = find_case(start, n, g);
   const CaseView& v = cv[k];
   const int c = (g - start[k]) * kBlock + threadIdx.x;
   if (c < v.m.nc)
-    grad_cell<kSecondOrder, Limiter, T>(v.m, v.nbr, static_cast<const T*>(v.V[vsel]), static_cast<T*>(v.grad),
-                                        static_cast<T*>(v.phi), c);
+    grad_cell<kSecondOrder, kFaceVals, Limiter, T>(v.m, v.nbr, static_cast<const T*>(v.V[vsel]),
+                                                   store_grad ? static_cast<T*>(v.grad) : nullptr,
+                                                   store_grad ? static_cast<T*>(v.phi) : nullptr,
+                                                   static_cast<T*>(v.fv), c);
 }
 
 template <class T>
@@ -918,7 +961,8 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kFluxWave
   double* Uacc = v.U[1 - cur];
   flux_tile<kSecondOrder, Visc, T>(v.m, v.t, g - start[k], reinterpret_cast<T*>(lds_raw),
                                    static_cast<const T*>(v.V[irk & 1]), static_cast<const T*>(v.grad),
-                                   static_cast<const T*>(v.phi), static_cast<const T*>(v.bc),
+                                   static_cast<const T*>(v.phi), static_cast<const T*>(v.fv),
+                                   static_cast<const T*>(v.bc),
                                    irk == 0 ? v.U[cur] : Uacc, Uacc, v.R, v.dt, beta, visc, v.U[cur],
                                    irk < 3 ? static_cast<T*>(v.V[1 - (irk & 1)]) : nullptr, alpha_next);
 }
@@ -985,6 +1029,7 @@ struct ma_solver {
   // Arithmetic-type buffers (double, or float for the FP32 form): primitives,
   // gradients, limiters, boundary contributions.
   void *V = nullptr, *V2 = nullptr, *grad = nullptr, *phi = nullptr, *bc = nullptr;
+  void* fv = nullptr;  // kMaxSlots x 5 planes: extrapolated face values (inviscid second order)
   bool fp32 = false;
   // Cases (independent meshes stepped together; a plain run is one case).
   int ncases = 1;
@@ -1813,8 +1858,22 @@ ma_status build_planned(ma_solver* s, RunPlan& P) {
     if (s->counting) {
       st = s->alloc(&dlr, ntf);
     } else {
-      std::vector<int2> h(ntf);
-      for (size_t i = 0; i < ntf; ++i) h[i] = make_int2(L.fl[L.tfaces[i]], L.fr[L.tfaces[i]]);
+      // slot of each internal face in its left and right cell's slot list,
+      // packed above the cell index for the face-value scheme (inviscid
+      // second order) only
+      const bool face_vals = s->opt.second_order_space && !s->opt.viscous;
+      std::vector<int> fs((size_t)2 * nif, 0);
+      for (int q = 0; face_vals && q < kMaxSlots; ++q)
+        for (int c = 0; c < nc; ++c) {
+          const int e = L.slot[(size_t)q * nc + c];
+          if (e >= 0 && (e >> 1) < nif) fs[(size_t)(e & 1) * nif + (e >> 1)] = q;
+        }
+      // (tfaces is padded to one entry when there are none; padding stays 0)
+      std::vector<int2> h(ntf, make_int2(0, 0));
+      for (size_t i = 0; i < (size_t)L.tptr[L.ntiles]; ++i) {
+        const int f = L.tfaces[i];
+        h[i] = make_int2(L.fl[f] | (fs[f] << kCellBits), L.fr[f] | (fs[(size_t)nif + f] << kCellBits));
+      }
       st = upload(s, &dlr, h);
     }
     if (st) return st;
@@ -1861,6 +1920,8 @@ ma_status build_planned(ma_solver* s, RunPlan& P) {
       (st = s->alloc_bytes(&s->V, n5 * tsize)) || (st = s->alloc_bytes(&s->V2, n5 * tsize)) ||
       (st = s->alloc_bytes(&s->grad, 3 * n5 * tsize)) ||
       (st = s->alloc_bytes(&s->phi, n5 * tsize)) ||
+      (s->opt.second_order_space && !s->opt.viscous &&
+       (st = s->alloc_bytes(&s->fv, (size_t)kMaxSlots * n5 * tsize))) ||
       (st = s->alloc_bytes(&s->bc, (size_t)5 * std::max<long long>(nbf, 1) * tsize)) ||
       (st = s->alloc(&s->partial, (size_t)std::max(1, s->t.ntiles))) || (st = s->alloc(&s->counter, 1)) ||
       (st = s->alloc(&s->norm_partial, (size_t)5 * std::max(1, s->t.ntiles))) ||
@@ -1905,11 +1966,17 @@ void run_stage(ma_solver* s, int irk) {
       hipLaunchKernelGGL(k_prim<T>, dim3(nb), dim3(kBlock), 0, s->stream, m.nc, Un, s->R, alpha[0], Vs);
     });
   }
+  // inviscid second order: extrapolated face values for the flux kernel;
+  // gradients and limiters are then stored on the last stage only (where
+  // the getters read them)
+  constexpr bool FV = SO && !Visc::isViscous;
   if (SO || Visc::isViscous) {
     s->launch(SO ? "k_grad_limiter" : "k_grad", [&] {
       const int per = (nb + kXcds - 1) / kXcds;
-      hipLaunchKernelGGL((k_grad<SO, Lim, T>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, m, s->nbr,
-                         static_cast<const T*>(Vs), static_cast<T*>(s->grad), static_cast<T*>(s->phi), per);
+      const bool store = !FV || irk == 3;
+      hipLaunchKernelGGL((k_grad<SO, FV, Lim, T>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, m, s->nbr,
+                         static_cast<const T*>(Vs), store ? static_cast<T*>(s->grad) : nullptr,
+                         store ? static_cast<T*>(s->phi) : nullptr, static_cast<T*>(s->fv), per);
     });
   }
   Visc visc{effective_prandtl(s->opt)};
@@ -1926,7 +1993,7 @@ void run_stage(ma_solver* s, int irk) {
   const size_t lds = (size_t)s->t.max_faces * 5 * sizeof(T);
   s->launch("k_flux", [&] {
     hipLaunchKernelGGL((k_flux<SO, Visc, T>), dim3(resident(s, s->t.per_xcd) * kXcds), dim3(kTile), lds, s->stream, m,
-                       s->t, static_cast<const T*>(Vs), static_cast<const T*>(s->grad),
+                       s->t, static_cast<const T*>(Vs), static_cast<const T*>(FV ? s->fv : s->grad),
                        static_cast<const T*>(s->phi), static_cast<const T*>(s->bc), acc_in, Uacc, s->R,
                        s->dt_dev, beta[irk], visc, Un, Vn, irk < 3 ? alpha[irk + 1] : 0.0);
   });
@@ -2669,6 +2736,7 @@ ma_status multi_set(Multi* M, ma_solver* const* cases, const int* parity, int n)
     v.V[1] = s->V2;
     v.grad = s->grad;
     v.phi = s->phi;
+    v.fv = s->fv;
     v.bc = s->bc;
     v.dt = s->dt_dev;
     v.time = s->time_dev;
@@ -2723,8 +2791,9 @@ void multi_step_t(ma::Multi* M, int parity) {
                      alpha[0], parity);
   for (int irk = 0; irk < 4; ++irk) {
     if (SO || Visc::isViscous)
-      hipLaunchKernelGGL((k_grad_multi<SO, Lim, T>), dim3(per_cells * kXcds), dim3(kBlock), 0, M->stream,
-                         M->d_views, sc, n, per_cells, irk & 1);
+      hipLaunchKernelGGL((k_grad_multi<SO, SO && !Visc::isViscous, Lim, T>), dim3(per_cells * kXcds),
+                         dim3(kBlock), 0, M->stream, M->d_views, sc, n, per_cells, irk & 1,
+                         (Visc::isViscous || irk == 3) ? 1 : 0);
     if (M->totals[1] > 0)
       hipLaunchKernelGGL(k_bc_multi<T>, dim3(M->totals[1]), dim3(kBlock), 0, M->stream, M->d_views, sb, n,
                          prandtl, irk & 1);
