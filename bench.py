@@ -30,14 +30,18 @@ HBM_PEAK_GBS = 8000.0
 
 
 def algorithmic_bytes(kernel: str, nc: int, nif: int, nbf: int, second_order: bool,
-                      t: int = 8, g: int = 8) -> float:
+                      t: int = 8, g: int = 8, face_values: bool = False) -> float:
     """Compulsory HBM bytes of one launch: every array the kernel needs read
     once, every output written once.  t = bytes of a primitive, gradient or
     limiter value (8 FP64, 4 in the FP32-arithmetic form), g = bytes of a
     geometry value (cell centroid and volume, face area/tangent/binormal,
     face centroid; also 4 in the FP32-arithmetic form); the state U, the
     RK accumulator and the residual are always FP64 (40 B per cell each);
-    connectivity is 8 B per face."""
+    connectivity is 8 B per face.  face_values: the inviscid second-order
+    scheme, where the gradient kernel writes each cell's extrapolated value
+    at each of its 6 faces (30 values per cell; gradients and limiters only
+    on the last of the 4 stages) and the flux kernel reads 5 of them per face
+    side instead of the cells' primitives, gradients and limiters."""
     F = nif + nbf
     if kernel == "k_flux":
         # V, grad, phi, x_c, vol, U_acc read and U_acc written per cell;
@@ -46,12 +50,16 @@ def algorithmic_bytes(kernel: str, nc: int, nif: int, nbf: int, second_order: bo
         # the four launches of a step, 110 + 3.75 t bytes per cell; per face
         # left/right, area/tangent/binormal, x_f
         per_cell = 110.0 + 3.75 * t
+        if face_values:
+            return (g + per_cell) * nc + (8 + 9 * g + 10 * t) * F
         if second_order:
             return (25 * t + 4 * g + per_cell) * nc + (8 + 12 * g) * F
         return (5 * t + g + per_cell) * nc + (8 + 9 * g) * F
     if kernel in ("k_grad", "k_grad_limiter"):
         # V, vol, x_c read; grad (+ phi) written; left/right, area, x_f per face
         vals = 25 if kernel == "k_grad_limiter" else 20
+        if face_values:
+            vals = 5 + 30 + 5  # V read; face values written; grad + phi on 1 stage in 4
         return (vals * t + 4 * g) * nc + (8 + 6 * g) * F
     if kernel == "k_prim":
         return (80.0 + 5 * t) * nc          # U_n, R read; V written
@@ -165,15 +173,16 @@ def timed_run(solver, steps: int, warmup: int, graph: int = 0):
     return time.perf_counter() - t0
 
 
-def roofline(solver, profile_steps, nc, nif, nbf, so, fp32, traffic, copy_gbs):
+def roofline(solver, profile_steps, nc, nif, nbf, so, fp32, traffic, copy_gbs, face_values=False):
     prof = solver.profile(profile_steps)
     t = g = 4 if fp32 else 8
     dom = max(prof, key=lambda k: prof[k][0] * prof[k][1])
     launches, mean_ms = prof[dom]
-    alg = algorithmic_bytes(dom, nc, nif, nbf, so, t, g)
+    alg = algorithmic_bytes(dom, nc, nif, nbf, so, t, g, face_values)
     achieved = alg / (mean_ms * 1e-3) / 1e9
     kernels = {k: {"launches": v[0], "mean_ms": round(v[1], 5),
-                   "GBps": round(algorithmic_bytes(k, nc, nif, nbf, so, t, g) / (v[1] * 1e-3) / 1e9, 1)}
+                   "GBps": round(algorithmic_bytes(k, nc, nif, nbf, so, t, g, face_values) / (v[1] * 1e-3) / 1e9,
+                                 1)}
                for k, v in prof.items()}
     print(f"[bench] kernels: {json.dumps(kernels)}", file=sys.stderr, flush=True)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -219,7 +228,8 @@ def bench_single(args, M, opts, copy_gbs) -> dict:
 
     so = bool(opts.second_order_space)
     roof = roofline(solver, args.profile_steps, nc, nif, nbf, so, opts.precision != 0,
-                    lambda k: load_traffic(args.workload, k), copy_gbs)
+                    lambda k: load_traffic(args.workload, k), copy_gbs,
+                    face_values=so and not opts.viscous)
     if opts.precision == 0:
         pb = path_bytes_per_step(nc, nif, nbf, so, bool(opts.viscous))
         path_gbs = pb / (elapsed / args.steps) / 1e9
@@ -284,7 +294,8 @@ def bench_batch(args, M, cases, copy_gbs) -> dict:
         print(f"[bench] precision error: {json.dumps(extra['precision_error'])}", file=sys.stderr, flush=True)
     del meshes
     roof = roofline(solver, args.profile_steps, nc, nif, nbf, True, fp32,
-                    lambda k: load_traffic(args.workload, k), copy_gbs)
+                    lambda k: load_traffic(args.workload, k), copy_gbs,
+                    face_values=bool(cases[0].second_order_space) and not cases[0].viscous)
     cpu = None
     if not args.no_cpu_baseline:
         try:
