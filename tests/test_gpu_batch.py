@@ -177,6 +177,37 @@ def test_tile_shape_does_not_change_results(shape, wg):
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("shape,wg", [((-1, 0, 0), 0), ((1, 1, 1), 0), ((3, 5, 7), 0), ((16, 4, 4), 2),
+                                      ((4, 8, 8), 3)])
+def test_face_value_scheme_tile_shapes(shape, wg):
+    """Inviscid second order (the gradient kernel stores each face's
+    extrapolated values; the tile lists carry each face's slot in its two
+    cells): any tile shape and resident-workgroup count gives the default
+    layout's bits, and those equal the oracle's."""
+    kw = dict(problem_type=0, lx=1.0, ly=0.25, lz=0.25, nx=64, ny=40, nz=36, dt=2e-5,
+              second_order_space=1, viscous=0, limiter=1, ic_kind=1, ic=(1.0, 1.0, 0.125, 0.1, 0.45),
+              bc_kind=(1, 1, 1, 1, 1, 1))
+    md, opts = make_case(kw)
+    out = []
+    for o in (opts, opts.replace(tile_shape=shape, workgroups_per_cu=wg)):
+        s = M.TimeSolverExplicitRK4(md, o)
+        try:
+            s.initialize()
+            s.step(4)
+            out.append(s.state())
+        finally:
+            s.close()
+    print(f"[gpu-batch] face values, tile shape {shape}, wg {wg}: bitwise={np.array_equal(out[0], out[1])}",
+          flush=True)
+    assert np.array_equal(out[0], out[1])
+    if shape == (3, 5, 7):
+        ok = O.make_options(**kw)
+        cpu = O.OracleSolver(O.OracleMesh(ok))
+        cpu.init()
+        cpu.step(4)
+        assert np.array_equal(out[1], cpu.state())
+
+
 def test_autotuned_batch_matches_default():
     kws = VANALBADA_CASES[:3]
     cases = [make_case(kw) for kw in kws]
