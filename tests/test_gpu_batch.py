@@ -177,6 +177,28 @@ def test_tile_shape_does_not_change_results(shape, wg):
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("shape,wg", [((-1, 0, 0), 0), ((1, 1, 1), 0), ((3, 5, 7), 0), ((2, 8, 16), 1)])
+def test_fp32_viscous_tiles_are_shape_invariant(shape, wg):
+    """The FP32 form is held to a tolerance against FP64, but against itself
+    it is deterministic: any tile shape and resident-workgroup count gives
+    the default layout's bits."""
+    kw = plate(nx=64, ny=40, nz=36, y_stretch=1.05, bc_kind=(3, 2, 2, 2, 1, 4), dt=0.0, cfl=0.5)
+    md, opts = make_case(kw)
+    opts = opts.replace(precision=1)
+    out = []
+    for o in (opts, opts.replace(tile_shape=shape, workgroups_per_cu=wg)):
+        s = M.TimeSolverExplicitRK4(md, o)
+        try:
+            s.initialize()
+            s.step(6)
+            out.append(s.state())
+        finally:
+            s.close()
+    print(f"[gpu-batch] fp32 viscous, tile shape {shape}, wg {wg}: bitwise={np.array_equal(out[0], out[1])}",
+          flush=True)
+    assert np.array_equal(out[0], out[1])
+
+
 @pytest.mark.parametrize("shape,wg", [((-1, 0, 0), 0), ((1, 1, 1), 0), ((3, 5, 7), 0), ((16, 4, 4), 2),
                                       ((4, 8, 8), 3)])
 def test_face_value_scheme_tile_shapes(shape, wg):
