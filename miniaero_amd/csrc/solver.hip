@@ -11,7 +11,9 @@
 //   k_prim   U_s = U_n + alpha_k R;  V = primitives(U_s)           (per cell, stage 0;
 //            stages 1-3 get V from the previous stage's k_flux)
 //   k_grad   Green-Gauss gradient, stencil min/max and the limiter  (per cell)
-//            of every cell, from V of the cell and its face neighbours
+//            of every cell, from V of the cell and its face neighbours;
+//            inviscid second order: also the cell's extrapolated values
+//            at its faces, which k_flux then reads instead of V, grad, phi
 //   k_bc     boundary-face contributions (Roe of the ghost state    (per boundary face)
 //            [+ no-slip viscous flux])
 //   k_flux   per tile of 256 cells: every internal face touching    (per cell tile)
