@@ -13,7 +13,9 @@ stream synchronisations.  The JSON line also carries:
                 workload, when present;
   cpu_baseline  the CPU oracle (a single-threaded port of the reference's
                 serial arithmetic) timed on a bounded sample of the same
-                physics on this host.
+                physics on this host's cores: one oracle process per core
+                (at most 16), all running the sample at once; the value is
+                their aggregate rate.
 """
 from __future__ import annotations
 
@@ -83,17 +85,13 @@ def path_bytes_per_step(nc: int, nif: int, nbf: int, second_order: bool, viscous
     return 4.0 * per
 
 
-def _oracle_rate(cases, steps: int):
-    """Cell-stage updates/s of the CPU oracle over the given options, run one
-    after another for `steps` RK4 steps each."""
-    import dataclasses
-
+def _oracle_run(cases, steps: int):
+    """The CPU oracle over the given option dicts, one after another for
+    `steps` RK4 steps each: (cells, seconds of stepping)."""
     from oracle import oracle as O
 
     cells, secs = 0, 0.0
-    for opts in cases:
-        kw = {f.name: getattr(opts, f.name) for f in dataclasses.fields(opts)
-              if f.name not in ("output_results", "output_frequency")}
+    for kw in cases:
         m = O.OracleMesh(O.make_options(**kw))
         s = O.OracleSolver(m)
         s.init()
@@ -101,7 +99,49 @@ def _oracle_rate(cases, steps: int):
         s.step(steps)
         secs += time.perf_counter() - t0
         cells += m.ncells
-    return cells * 4 * steps / secs, cells, secs
+    return cells, secs
+
+
+def _cpu_cores() -> int:
+    """Host cores this process may run on, at most 16 (one GPU's share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _oracle_rate(cases, steps: int):
+    """Aggregate cell-stage updates/s of the CPU oracle on all usable host
+    cores: one single-threaded oracle process per core (started as child
+    processes, which touch no GPU), each running every case of `cases` for
+    `steps` RK4 steps; total updates over the slowest process's stepping
+    time.  Returns (rate, cells per process, seconds, processes)."""
+    import dataclasses
+    import subprocess
+
+    kws = [{f.name: getattr(o, f.name) for f in dataclasses.fields(o)
+            if f.name not in ("output_results", "output_frequency")} for o in cases]
+    spec = json.dumps({"cases": kws, "steps": steps})
+    n = _cpu_cores()
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker"], stdin=subprocess.PIPE,
+                              stdout=subprocess.PIPE, text=True) for _ in range(n)]
+    outs = [p.communicate(spec)[0] for p in procs]
+    if any(p.returncode != 0 for p in procs):
+        raise RuntimeError("an oracle worker failed")
+    res = [json.loads(o.strip().splitlines()[-1]) for o in outs]
+    cells = res[0]["cells"]
+    secs = max(r["secs"] for r in res)
+    return n * cells * 4 * steps / secs, cells, secs, n
+
+
+def cpu_worker() -> int:
+    """--cpu-worker: run the oracle on the cases given as JSON on stdin and
+    print {"cells", "secs"}."""
+    spec = json.loads(sys.stdin.read())
+    cells, secs = _oracle_run(spec["cases"], spec["steps"])
+    print(json.dumps({"cells": cells, "secs": secs}), flush=True)
+    return 0
 
 
 def cpu_baseline(opts) -> dict:
@@ -110,11 +150,11 @@ def cpu_baseline(opts) -> dict:
     RK4 steps."""
     steps = 5
     sample = opts.replace(nx=max(1, opts.nx // 2), ny=max(1, opts.ny // 2), nz=max(1, opts.nz // 2))
-    rate, cells, secs = _oracle_rate([sample], steps)
-    return {"value": rate, "unit": "cell-stage updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (single-thread C++ restatement of the reference's serial path), "
-                      f"{sample.nx}x{sample.ny}x{sample.nz} = {cells} cells, {steps} RK4 steps, "
-                      f"same physics/options, {secs:.1f} s"}
+    rate, cells, secs, n = _oracle_rate([sample], steps)
+    return {"value": rate, "unit": "cell-stage updates/s", "cores": n, "kind": "port",
+            "sample": f"oracle (single-thread C++ restatement of the reference's serial path), one process "
+                      f"per core on {n} cores, each {sample.nx}x{sample.ny}x{sample.nz} = {cells} cells, "
+                      f"{steps} RK4 steps, same physics/options, {secs:.1f} s"}
 
 
 def cpu_baseline_batch(cases, min_cells: int = 600_000) -> dict:
@@ -127,11 +167,11 @@ def cpu_baseline_batch(cases, min_cells: int = 600_000) -> dict:
         n += o.nx * o.ny * o.nz
         if n >= min_cells:
             break
-    rate, cells, secs = _oracle_rate(chosen, steps)
-    return {"value": rate, "unit": "cell-stage updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (single-thread C++ restatement of the reference's serial path), "
-                      f"the first {len(chosen)} cases of the batch ({cells} cells), {steps} RK4 steps "
-                      f"each, {secs:.1f} s"}
+    rate, cells, secs, n = _oracle_rate(chosen, steps)
+    return {"value": rate, "unit": "cell-stage updates/s", "cores": n, "kind": "port",
+            "sample": f"oracle (single-thread C++ restatement of the reference's serial path), one process "
+                      f"per core on {n} cores, each running the first {len(chosen)} cases of the batch "
+                      f"({cells} cells), {steps} RK4 steps each, {secs:.1f} s"}
 
 
 def load_traffic(workload: str, kernel: str):
@@ -445,7 +485,10 @@ def main() -> int:
                     help="resident workgroups per CU for the flux kernel (0: one per tile)")
     ap.add_argument("--copy-ceiling", action="store_true",
                     help="only measure the plain copy kernel (1 GiB, read + write) and exit")
+    ap.add_argument("--cpu-worker", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_worker:
+        return cpu_worker()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it "
                                    "(cases and mesh blocks shard across GPUs as described in DESIGN.md)"}))
