@@ -124,9 +124,16 @@ def _oracle_rate(cases, steps: int):
             if f.name not in ("output_results", "output_frequency")} for o in cases]
     spec = json.dumps({"cases": kws, "steps": steps})
     n = _cpu_cores()
+    print(f"[bench] cpu baseline: {n} oracle processes, {len(cases)} case(s) x {steps} steps each",
+          file=sys.stderr, flush=True)
     procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker"], stdin=subprocess.PIPE,
                               stdout=subprocess.PIPE, text=True) for _ in range(n)]
-    outs = [p.communicate(spec)[0] for p in procs]
+    for p in procs:  # every worker gets its input before any output is awaited, so all run at once
+        p.stdin.write(spec)
+        p.stdin.close()
+    outs = [p.stdout.read() for p in procs]
+    for p in procs:
+        p.wait()
     if any(p.returncode != 0 for p in procs):
         raise RuntimeError("an oracle worker failed")
     res = [json.loads(o.strip().splitlines()[-1]) for o in outs]
