@@ -316,6 +316,7 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
   T vc[5];
   load5(V, nc, c, vc);
   const T vol = T(m.vol[c]);
+  const T rvol = T(1) / vol;
 
   T g[5][3];
 #pragma unroll
@@ -349,7 +350,13 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
         const T vl = side ? vn[k] : vc[k];
         const T vr = side ? vc[k] : vn[k];
         const T gg = ok ? T(0.5) * (vl + vr) * n[d] : T(0);
-        g[k][d] += side ? -gg / vol : gg / vol;
+        // FP32 viscous form: one reciprocal per cell (k_grad -6%; in the
+        // face-value scheme it measured 2% slower, so that keeps the division);
+        // FP64: the reference's division per face
+        if (sizeof(T) == 4 && !kFaceVals)
+          g[k][d] += (side ? -gg : gg) * rvol;
+        else
+          g[k][d] += side ? -gg / vol : gg / vol;
       }
     if (kSecondOrder) {
 #pragma unroll
