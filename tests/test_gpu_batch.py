@@ -119,12 +119,14 @@ def test_many_small_cases():
     run_batch_vs_alone(kws, 6)
 
 
-@pytest.mark.parametrize("graph_steps", [2, 4])
-def test_graph_replays_match_plain_launches(graph_steps):
+@pytest.mark.parametrize("graph_steps,scheme", [(2, "viscous"), (4, "viscous"), (2, "inviscid")])
+def test_graph_replays_match_plain_launches(graph_steps, scheme):
     """Steps run as captured HIP graphs (several replays, from both ping-pong
     parities, with eager work and host reads between them) equal plain
-    launches bit for bit, for a batch with fixed-dt and CFL cases."""
-    kws = [VISCOUS_CASES[1], VISCOUS_CASES[2], VISCOUS_CASES[4]]
+    launches bit for bit, for a batch with fixed-dt and CFL cases (viscous)
+    and for the inviscid second-order face-value scheme."""
+    kws = ([VISCOUS_CASES[1], VISCOUS_CASES[2], VISCOUS_CASES[4]] if scheme == "viscous"
+           else VANALBADA_CASES[:3])
     cases = [make_case(kw) for kw in kws]
     a = M.BatchSolver(cases)
     b = M.BatchSolver(cases)
