@@ -297,7 +297,10 @@ __global__ __launch_bounds__(kBlock) void k_prim(int nc, const double* __restric
 // min(v, v) == v exactly -- so the slot loop has no branch; all six slots'
 // area vectors and neighbour values are loaded before the first is used (one
 // round of gathers per wavefront instead of one per face, measured 8-16%
-// faster than the per-face form), and empty slots are masked.
+// faster than the per-face form), and empty slots are masked: an empty slot
+// adds +-0 to the gradient sums, which leaves them unchanged (a sum starts at
+// +0 and is -0 only when both terms are, so it is never -0), and leaves the
+// min/max and the limiter as they were.
 // kFaceVals (inviscid second order): each slot's reconstructed value -- the
 // extrapolation v + phi ((x_f - x_c) . grad) of Flux.h:107-130, the same
 // expression in the same order as the flux kernel's otherwise -- is stored
