@@ -304,3 +304,24 @@ def test_cpp_many_cases_program_reports_missing_device(tmp_path):
     assert os.path.exists(exe), "bin/many_cases not built"
     r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "no HIP device" in r.stderr
+
+
+def test_bench_algorithmic_bytes_match_the_stated_figures():
+    """The roofline's algorithmic bytes per launch: the per-cell and per-face
+    figures the kernel table states, on the 256x128x64 plate (FP64) and in
+    the FP32 and face-value forms, and the survey's whole-step figure."""
+    import sys
+    sys.path.insert(0, ROOT)
+    import bench
+    nc, nif, nbf = 2097152, 6234112, 114688
+    F = nif + nbf
+    assert bench.algorithmic_bytes("k_flux", nc, nif, nbf, True, 8, 8) == 372 * nc + 104 * F
+    assert bench.algorithmic_bytes("k_flux", nc, nif, nbf, True, 4, 4) == 241 * nc + 56 * F
+    assert abs(bench.algorithmic_bytes("k_flux", nc, nif, nbf, True) - 1.4404e9) < 1e5
+    assert bench.algorithmic_bytes("k_grad_limiter", nc, nif, nbf, True) == 232 * nc + 56 * F
+    # face-value scheme: 5 values per face side instead of the cells' 25
+    assert bench.algorithmic_bytes("k_flux", nc, nif, nbf, True, 4, 4, True) == (4 + 125) * nc + (8 + 36 + 40) * F
+    assert bench.algorithmic_bytes("k_grad_limiter", nc, nif, nbf, True, 4, 4, True) == (160 + 16) * nc + 32 * F
+    assert bench.algorithmic_bytes("k_prim", nc, nif, nbf, True) == 120 * nc
+    assert bench.path_bytes_per_step(nc, nif, nbf, True, True) == 4 * (824 * nc + 168 * F)
+    assert abs(bench.path_bytes_per_step(nc, nif, nbf, True, True) / (4 * nc) - 1333) < 1
