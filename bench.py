@@ -48,10 +48,11 @@ def algorithmic_bytes(kernel: str, nc: int, nif: int, nbf: int, second_order: bo
     if kernel == "k_flux":
         # V, grad, phi, x_c, vol, U_acc read and U_acc written per cell;
         # stages 0-2 also read U_n (the same array as U_acc's input on stage
-        # 0) and write the next stage's V, stage 3 writes R: averaged over
-        # the four launches of a step, 110 + 3.75 t bytes per cell; per face
-        # left/right, area/tangent/binormal, x_f
-        per_cell = 110.0 + 3.75 * t
+        # 0), every stage writes the next stage's V (stage 3 the next
+        # step's), stage 3 writes R: averaged over the four launches of a
+        # step, 110 + 5 t bytes per cell; per face left/right,
+        # area/tangent/binormal, x_f
+        per_cell = 110.0 + 5.0 * t
         if face_values:
             return (g + per_cell) * nc + (8 + 9 * g + 10 * t) * F
         if second_order:

@@ -8,8 +8,9 @@
 // launches and fences per second-order stage -- with three cell-centric
 // kernels and no atomics:
 //
-//   k_prim   U_s = U_n + alpha_k R;  V = primitives(U_s)           (per cell, stage 0;
-//            stages 1-3 get V from the previous stage's k_flux)
+//   k_prim   U_s = U_n + alpha_k R;  V = primitives(U_s)           (per cell, stage 0 when
+//            V does not hold them; otherwise every stage's V comes from
+//            the previous stage's k_flux)
 //   k_grad   Green-Gauss gradient, stencil min/max and the limiter  (per cell)
 //            of every cell, from V of the cell and its face neighbours;
 //            inviscid second order: also the cell's extrapolated values
@@ -20,8 +21,9 @@
 //            the tile evaluated once into LDS (Roe [+ reconstruction]
 //            [+ viscous]), then per cell: sum of face and boundary
 //            contributions, residual R = dt/V sum, U_acc += beta_k R,
-//            and (stages 0-2) the next stage's V from U_n + alpha_k+1 R
-//            into the other V buffer (R itself is stored on stage 3 only)
+//            and the next stage's V from U_n + alpha_k+1 R into the other
+//            V buffer (stage 3: the next step's, from the new U; R itself
+//            is stored on stage 3 only)
 //
 // Determinism and parity.  The reference scatters face results into cells
 // with atomics; with one thread the additions land in a fixed order: internal
@@ -601,17 +603,20 @@ __global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const T* __restrict__ 
 // internal face (index = LDS slot), 4 + kind - 1 = boundary face of that kind
 // (index = boundary-face number).
 template <bool kSecondOrder, class Visc, class T>
-// Un, Vnext, alpha_next: with Vnext set (stages 0-2) the cell's primitives
-// for the next stage, primitives(Un + alpha_next R) -- update<Device> and
+// Un, Vnext, alpha_next, last: with Vnext set, the cell's primitives for the
+// next stage, primitives(base + alpha_next R) -- update<Device> and
 // ComputePrimitives of that stage, the same arithmetic as k_prim -- are
-// written to Vnext and R is not stored; on the last stage (Vnext null) R is.
+// written to Vnext, base being Un on stages 0-2 and, on the last stage, the
+// new state just written to Uacc_out (the next step's stage 0, alpha 0).
+// R itself is stored on the last stage only.
 __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, int tile, T* lds,
                                           const T* __restrict__ V, const T* __restrict__ grad,
                                           const T* __restrict__ phi, const T* __restrict__ fv,
                                           const T* __restrict__ bc,
                                           const double* Uacc_in, double* Uacc_out, double* __restrict__ R,
                                           const double* __restrict__ dt_dev, double beta, const Visc& visc,
-                                          const double* __restrict__ Un, T* __restrict__ Vnext, double alpha_next) {
+                                          const double* __restrict__ Un, T* __restrict__ Vnext, double alpha_next,
+                                          int last) {
   const int f0 = t.ptr[tile];
   const int nfl = t.ptr[tile + 1] - f0;
   const int mf = t.max_faces;
@@ -649,7 +654,7 @@ __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, in
   // need not wait behind the stores.
   double uin[5], un[5];
   load5(Uacc_in, nc, c, uin);
-  if (Vnext) load5(Un, nc, c, un);
+  if (Vnext && !last) load5(Un, nc, c, un);
   const double dt = dt_dev[t.tcase[tile]];
   const double vol = m.vol[c];
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -685,22 +690,24 @@ __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, in
         for (int k = 0; k < 5; ++k) acc[k] += double(bc[(size_t)k * nbf + j]);
       }
   }
-  double rr[5];
+  double rr[5], unew[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
     double r = 0.0;
     r = r + dt / vol * acc[k];
     rr[k] = r;
-    Uacc_out[(size_t)k * nc + c] = uin[k] + beta * r;
+    unew[k] = uin[k] + beta * r;
+    Uacc_out[(size_t)k * nc + c] = unew[k];
   }
   if (Vnext) {
     double u[5], v[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) u[k] = un[k] + alpha_next * rr[k];
+    for (int k = 0; k < 5; ++k) u[k] = (last ? unew[k] : un[k]) + alpha_next * rr[k];
     ma::primitives(u, v);
 #pragma unroll
     for (int k = 0; k < 5; ++k) Vnext[(size_t)k * nc + c] = T(v[k]);
-  } else {
+  }
+  if (last) {
 #pragma unroll
     for (int k = 0; k < 5; ++k) R[(size_t)k * nc + c] = rr[k];
   }
@@ -721,7 +728,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kFluxWave
                                                 double* Uacc_out, double* __restrict__ R,
                                                 const double* __restrict__ dt_dev, double beta,
                                                 Visc visc, const double* __restrict__ Un, T* Vnext,
-                                                double alpha_next) {
+                                                double alpha_next, int last) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* lds = reinterpret_cast<T*>(lds_raw);  // 5 planes of t.max_faces
   // XCD-aware order: the workgroups an XCD receives (every kXcds-th) take a
@@ -735,7 +742,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kFluxWave
     if (tile >= t.ntiles) break;  // uniform over the workgroup
     // the face-value scheme reads no gradients: its face values come in grad
     flux_tile<kSecondOrder, Visc, T>(m, t, tile, lds, V, grad, phi, grad, bc, Uacc_in, Uacc_out, R, dt_dev,
-                                     beta, visc, Un, Vnext, alpha_next);
+                                     beta, visc, Un, Vnext, alpha_next, last);
     __syncthreads();  // the LDS is reused by the next tile
   }
 }
@@ -976,7 +983,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kFluxWave
                                    static_cast<const T*>(v.phi), static_cast<const T*>(v.fv),
                                    static_cast<const T*>(v.bc),
                                    irk == 0 ? v.U[cur] : Uacc, Uacc, v.R, v.dt, beta, visc, v.U[cur],
-                                   irk < 3 ? static_cast<T*>(v.V[1 - (irk & 1)]) : nullptr, alpha_next);
+                                   irk < 3 ? static_cast<T*>(v.V[1 - (irk & 1)]) : nullptr, alpha_next, irk == 3);
 }
 
 __global__ __launch_bounds__(kTile) void k_dt_partial_multi(const CaseView* __restrict__ cv,
@@ -1059,6 +1066,9 @@ struct ma_solver {
   const int* org = nullptr;    // device cell -> caller's cell index
   const int* nbr = nullptr;    // kMaxSlots planes of nc: the cell across each slot (grad_cell)
   int cur = 0;  // U[cur] holds sol_n
+  // V holds primitives(U[cur]) (written by the last stage of the previous
+  // single-case step); false after initialize, set_state and multi-case steps
+  bool v0_valid = false;
   long long steps_done = 0;
   // HIP graphs of graph_steps whole RK4 steps (an even count, so U[cur] is
   // sol_n again after a replay), one per starting value of cur.
@@ -1971,9 +1981,11 @@ void run_stage(ma_solver* s, int irk) {
   double* Uacc = s->U[1 - s->cur];
   // the stage's primitives: V for stages 0 and 2, V2 for 1 and 3; stage 0's
   // come from k_prim, the others from the previous stage's flux kernel
+  // (stage 3 writes the next step's stage-0 primitives into V, so k_prim
+  // runs only when V does not hold them yet)
   T* Vs = static_cast<T*>((irk & 1) ? s->V2 : s->V);
-  T* Vn = irk < 3 ? static_cast<T*>((irk & 1) ? s->V : s->V2) : nullptr;
-  if (irk == 0) {
+  T* Vn = static_cast<T*>((irk & 1) ? s->V : s->V2);
+  if (irk == 0 && !s->v0_valid) {
     s->launch("k_prim", [&] {
       hipLaunchKernelGGL(k_prim<T>, dim3(nb), dim3(kBlock), 0, s->stream, m.nc, Un, s->R, alpha[0], Vs);
     });
@@ -2007,8 +2019,9 @@ void run_stage(ma_solver* s, int irk) {
     hipLaunchKernelGGL((k_flux<SO, Visc, T>), dim3(resident(s, s->t.per_xcd) * kXcds), dim3(kTile), lds, s->stream, m,
                        s->t, static_cast<const T*>(Vs), static_cast<const T*>(FV ? s->fv : s->grad),
                        static_cast<const T*>(s->phi), static_cast<const T*>(s->bc), acc_in, Uacc, s->R,
-                       s->dt_dev, beta[irk], visc, Un, Vn, irk < 3 ? alpha[irk + 1] : 0.0);
+                       s->dt_dev, beta[irk], visc, Un, Vn, irk < 3 ? alpha[irk + 1] : alpha[0], irk == 3 ? 1 : 0);
   });
+  if (irk == 3) s->v0_valid = true;  // V now holds the next step's stage-0 primitives
 }
 
 template <bool SO, class Visc, class Lim, class T>
@@ -2083,11 +2096,30 @@ ma_status capture_graph(ma_solver* s) {
   return MA_OK;
 }
 
+// The stage-0 primitives of U[cur] into V, when V does not hold them.
+ma_status ensure_stage0_prims(ma_solver* s) {
+  if (s->v0_valid) return MA_OK;
+  const int nc = s->m.nc;
+  const double* Un = s->U[s->cur];
+  if (s->fp32)
+    hipLaunchKernelGGL(k_prim<float>, dim3(nblocks(nc)), dim3(kBlock), 0, s->stream, nc, Un, s->R, 0.0,
+                       static_cast<float*>(s->V));
+  else
+    hipLaunchKernelGGL(k_prim<double>, dim3(nblocks(nc)), dim3(kBlock), 0, s->stream, nc, Un, s->R, 0.0,
+                       static_cast<double*>(s->V));
+  MA_HIP(hipGetLastError());
+  s->v0_valid = true;
+  return MA_OK;
+}
+
 // nsteps steps: whole graphs while at least graph_steps remain, the rest as
-// plain launches.
+// plain launches.  A graph starts from valid stage-0 primitives (made here
+// when needed), so it never contains k_prim and leaves them valid.
 ma_status advance(ma_solver* s, int nsteps) {
   if (s->graph_steps > 0) {
     while (nsteps >= s->graph_steps) {
+      ma_status st0 = ensure_stage0_prims(s);
+      if (st0) return st0;
       if (!s->graph[s->cur]) {
         ma_status st = capture_graph(s);
         if (st) return st;
@@ -2320,6 +2352,7 @@ extern "C" ma_status ma_solver_initialize(ma_solver* s) {
   MA_HIP(hipMemsetAsync(s->time_dev, 0, s->ncases * sizeof(double), s->stream));
   MA_HIP(hipStreamSynchronize(s->stream));
   s->steps_done = 0;
+  s->v0_valid = false;
   return MA_OK;
 }
 
@@ -2333,6 +2366,7 @@ extern "C" ma_status ma_solver_set_state(ma_solver* s, const double* U) {
   MA_HIP(hipMemcpyAsync(s->U[0], tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
   MA_HIP(hipMemsetAsync(s->R, 0, tmp.size() * sizeof(double), s->stream));
   MA_HIP(hipStreamSynchronize(s->stream));
+  s->v0_valid = false;
   return MA_OK;
 }
 
@@ -2849,6 +2883,7 @@ ma_status multi_step(Multi* M, int parity) {
 void solver_set_progress(ma_solver* s, int steps) {
   s->cur ^= (steps & 1);
   s->steps_done += steps;
+  s->v0_valid = false;  // the multi-case kernels do not write the stage-0 primitives
 }
 
 int solver_cur(const ma_solver* s) { return s->cur; }

@@ -305,3 +305,33 @@ def test_cpp_facade_batched_and_served_cases(tmp_path):
     r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=240)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0 and r.stdout.strip().endswith("OK")
+
+
+@pytest.mark.parametrize("graph_steps", [0, 2])
+def test_restart_from_a_state_equals_the_uninterrupted_run(graph_steps):
+    """set_state after other steps (so the solver holds the primitives of a
+    different state), then steps -- plain or as graph replays -- equal the
+    uninterrupted run from that state bit for bit (CFL time step: dt comes
+    from the state alone)."""
+    for kw in (VISCOUS_CASES[4], VANALBADA_CASES[0]):
+        md, opts = make_case(kw)
+        a = M.TimeSolverExplicitRK4(md, opts)
+        b = M.TimeSolverExplicitRK4(md, opts)
+        try:
+            a.initialize()
+            a.step(3)
+            U3 = a.state()
+            a.step(4)
+            U7 = a.state()
+            if graph_steps:
+                b.set_graph(graph_steps)
+            b.initialize()
+            b.step(2)
+            b.set_state(U3)
+            b.step(4)
+            same = np.array_equal(b.state(), U7)
+            print(f"[gpu-batch] restart, graph {graph_steps}, viscous {opts.viscous}: bitwise={same}", flush=True)
+            assert same
+        finally:
+            a.close()
+            b.close()
