@@ -720,15 +720,16 @@ template <class T>
 constexpr int kFluxWaves = sizeof(T) == 8 ? 3 : 1;
 
 template <bool kSecondOrder, class Visc, class T>
-__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kFluxWaves<T>))) void k_flux(DevMesh m, TileMap t, const T* __restrict__ V,
-                                                const T* __restrict__ grad,
-                                                const T* __restrict__ phi,
-                                                const T* __restrict__ bc,
-                                                const double* Uacc_in,  // may alias Uacc_out
-                                                double* Uacc_out, double* __restrict__ R,
-                                                const double* __restrict__ dt_dev, double beta,
-                                                Visc visc, const double* __restrict__ Un, T* Vnext,
-                                                double alpha_next, int last) {
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kFluxWaves<T>)))
+void k_flux(DevMesh m, TileMap t, const T* __restrict__ V,
+            const T* __restrict__ grad,
+            const T* __restrict__ phi,
+            const T* __restrict__ bc,
+            const double* Uacc_in,  // may alias Uacc_out
+            double* Uacc_out, double* __restrict__ R,
+            const double* __restrict__ dt_dev, double beta,
+            Visc visc, const double* __restrict__ Un, T* Vnext,
+            double alpha_next, int last) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* lds = reinterpret_cast<T*>(lds_raw);  // 5 planes of t.max_faces
   // XCD-aware order: the workgroups an XCD receives (every kXcds-th) take a
@@ -968,9 +969,10 @@ __global__ __launch_bounds__(kBlock) void k_bc_multi(const CaseView* __restrict_
 }
 
 template <bool kSecondOrder, class Visc, class T>
-__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kFluxWaves<T>))) void k_flux_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
-                                                      int n, int per_xcd, int parity, int irk, double beta,
-                                                      double alpha_next, Visc visc) {
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kFluxWaves<T>)))
+void k_flux_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
+                  int n, int per_xcd, int parity, int irk, double beta,
+                  double alpha_next, Visc visc) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int g = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
   if (g >= start[n]) return;  // whole workgroup, before any barrier
