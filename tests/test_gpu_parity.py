@@ -166,6 +166,36 @@ def test_gradients_limiters_and_residual_match():
         p.close()
 
 
+@pytest.mark.parametrize("name", ["plate_cfl_stretch", "sod_vanalbada", "perturbed_first_order_viscous"])
+def test_extreme_exponent_gradients_match(name):
+    """Momenta near the bottom of the FP64 range (1e-306, and subnormal) in
+    every 7th cell, beside exact zeros: gradient quotients, limiter ratios
+    and fluxes then meet subnormal operands and results, which the GPU keeps
+    (no flush to zero) as the CPU does.  State, gradients and limiters stay
+    bit-identical."""
+    kw = dict(CONFIG_CASES[name], ntimesteps=2)
+    p = Pair(kw)
+    try:
+        U = p.gpu.state().copy()
+        idx = np.arange(0, U.shape[0], 7)
+        U[idx, 2] = U[idx, 0] * 3e-306
+        U[idx[::2], 3] = U[idx[::2], 0] * 2.5e-310
+        assert np.count_nonzero((U[:, 3] != 0) & (np.abs(U[:, 3]) < 2.2250738585072014e-308)) > 0
+        p.gpu.set_state(U)
+        p.cpu.set_state(U)
+        for i in range(2):
+            G, R = p.step(1)
+            _log(f"  step {i + 1}: bitwise={np.array_equal(G, R)}")
+            assert np.array_equal(G, R), f"step {i + 1} not bit-identical"
+        g, lim = p.gpu.gradients()
+        og, olim = p.cpu.gradients()
+        assert np.array_equal(g, og)
+        if kw["second_order_space"]:
+            assert np.array_equal(lim, olim)
+    finally:
+        p.close()
+
+
 def test_deterministic_and_finite():
     kw = dict(_case("FlatPlate_Parallel"), ntimesteps=20)
     a = Pair(kw)
