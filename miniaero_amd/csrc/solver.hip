@@ -918,6 +918,7 @@ struct CaseView {
   const int* bc_case;
   const int* nbr;  // neighbour-cell planes (grad_cell)
   int parity;  // U[parity ^ step parity] holds the case's sol_n
+  int prim0;   // 1: V[0] does not hold the primitives of the case's sol_n yet
 };
 
 // Prefix table: start[k] = first global block of case k, start[n] = total.
@@ -931,13 +932,16 @@ __device__ __forceinline__ int find_case(const int* __restrict__ start, int n, i
   return lo;
 }
 
-// Stage 0 only (the flux kernel of stages 0-2 writes the next stage's primitives).
+// Stage 0, for the cases whose V[0] is not current (new in the group; the
+// flux kernel of every stage writes the next stage's primitives, the last
+// stage the next step's).
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_prim_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
                                                        int n, double alpha, int parity) {
   const int g = blockIdx.x;
   const int k = find_case(start, n, g);
   const CaseView& v = cv[k];
+  if (!v.prim0) return;
   const int c = (g - start[k]) * kBlock + threadIdx.x;
   if (c < v.m.nc) prim_cell(v.m.nc, v.U[v.parity ^ parity], v.R, alpha, static_cast<T*>(v.V[0]), c);
 }
@@ -985,7 +989,7 @@ void k_flux_multi(const CaseView* __restrict__ cv, const int* __restrict__ start
                                    static_cast<const T*>(v.phi), static_cast<const T*>(v.fv),
                                    static_cast<const T*>(v.bc),
                                    irk == 0 ? v.U[cur] : Uacc, Uacc, v.R, v.dt, beta, visc, v.U[cur],
-                                   irk < 3 ? static_cast<T*>(v.V[1 - (irk & 1)]) : nullptr, alpha_next, irk == 3);
+                                   static_cast<T*>(v.V[1 - (irk & 1)]), alpha_next, irk == 3);
 }
 
 __global__ __launch_bounds__(kTile) void k_dt_partial_multi(const CaseView* __restrict__ cv,
@@ -1069,7 +1073,7 @@ struct ma_solver {
   const int* nbr = nullptr;    // kMaxSlots planes of nc: the cell across each slot (grad_cell)
   int cur = 0;  // U[cur] holds sol_n
   // V holds primitives(U[cur]) (written by the last stage of the previous
-  // single-case step); false after initialize, set_state and multi-case steps
+  // step, single- or multi-case); false after initialize and set_state
   bool v0_valid = false;
   long long steps_done = 0;
   // HIP graphs of graph_steps whole RK4 steps (an even count, so U[cur] is
@@ -2722,6 +2726,8 @@ struct Multi {
   int slot = 0;
   int totals[3] = {0, 0, 0};
   int max_faces = 1;
+  std::vector<ma_solver*> cases;  // the group's solvers, in table order
+  bool any_prim0 = false;         // some case needs its stage-0 primitives computed
 };
 
 ma_status multi_create(const ma_options& scheme, int capacity, hipStream_t stream, Multi** out) {
@@ -2771,6 +2777,8 @@ ma_status multi_set(Multi* M, ma_solver* const* cases, const int* parity, int n)
   st[0] = st[cap1] = st[2 * cap1] = 0;
   M->any_cfl = false;
   M->max_faces = 1;
+  M->any_prim0 = false;
+  M->cases.assign(cases, cases + n);
   for (int k = 0; k < n; ++k) {
     ma_solver* s = cases[k];
     if (s->ncases != 1 || s->fp32 != M->fp32) return ma::fail(MA_ERR_INVALID, "multi: incompatible case");
@@ -2795,6 +2803,8 @@ ma_status multi_set(Multi* M, ma_solver* const* cases, const int* parity, int n)
     v.bc_case = s->bc_case;
     v.nbr = s->nbr;
     v.parity = parity[k];
+    v.prim0 = s->v0_valid ? 0 : 1;
+    M->any_prim0 = M->any_prim0 || !s->v0_valid;
     st[k + 1] = st[k] + nblocks(s->m.nc);
     st[cap1 + k + 1] = st[cap1 + k] + nblocks(s->m.nf - s->m.nif);
     st[2 * cap1 + k + 1] = st[2 * cap1 + k] + s->t.ntiles;
@@ -2835,8 +2845,9 @@ void multi_step_t(ma::Multi* M, int parity) {
   const int per_cells = (M->totals[0] + kXcds - 1) / kXcds;
   const int per_tiles = (M->totals[2] + kXcds - 1) / kXcds;
   const size_t lds = (size_t)M->max_faces * 5 * sizeof(T);
-  hipLaunchKernelGGL(k_prim_multi<T>, dim3(M->totals[0]), dim3(kBlock), 0, M->stream, M->d_views, sc, n,
-                     alpha[0], parity);
+  if (M->any_prim0)
+    hipLaunchKernelGGL(k_prim_multi<T>, dim3(M->totals[0]), dim3(kBlock), 0, M->stream, M->d_views, sc, n,
+                       alpha[0], parity);
   for (int irk = 0; irk < 4; ++irk) {
     if (SO || Visc::isViscous)
       hipLaunchKernelGGL((k_grad_multi<SO, SO && !Visc::isViscous, Lim, T>), dim3(per_cells * kXcds),
@@ -2879,13 +2890,16 @@ ma_status multi_step(Multi* M, int parity) {
   else
     multi_t<double>(M, parity);
   MA_HIP(hipGetLastError());
+  // The last stage wrote every case's next stage-0 primitives (stream order).
+  M->any_prim0 = false;
+  for (ma_solver* s : M->cases) s->v0_valid = true;
   return MA_OK;
 }
 
 void solver_set_progress(ma_solver* s, int steps) {
   s->cur ^= (steps & 1);
   s->steps_done += steps;
-  s->v0_valid = false;  // the multi-case kernels do not write the stage-0 primitives
+  s->v0_valid = true;  // the multi-case flux kernel's last stage wrote the stage-0 primitives
 }
 
 int solver_cur(const ma_solver* s) { return s->cur; }
