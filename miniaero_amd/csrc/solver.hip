@@ -1098,6 +1098,10 @@ struct ma_solver {
   // V holds primitives(U[cur]) (written by the last stage of the previous
   // step, single- or multi-case); false after initialize and set_state
   bool v0_valid = false;
+  // inviscid second order: the last stage's gradients and limiters are not in
+  // grad/phi (the flux kernel reads face values); its primitives are in V2,
+  // from which the getter computes them
+  bool grads_stale = false;
   long long steps_done = 0;
   // HIP graphs of graph_steps whole RK4 steps (an even count, so U[cur] is
   // sol_n again after a replay), one per starting value of cur.
@@ -2020,13 +2024,13 @@ void run_stage(ma_solver* s, int irk) {
     });
   }
   // inviscid second order: extrapolated face values for the flux kernel;
-  // gradients and limiters are then stored on the last stage only (where
-  // the getters read them)
+  // gradients and limiters are then not stored (the getter computes the last
+  // stage's from V2, see fill_gradients)
   constexpr bool FV = SO && !Visc::isViscous;
   if (SO || Visc::isViscous) {
     s->launch(SO ? "k_grad_limiter" : "k_grad", [&] {
       const int per = (nb + kXcds - 1) / kXcds;
-      const bool store = !FV || irk == 3;
+      const bool store = !FV;
       hipLaunchKernelGGL((k_grad<SO, FV, Lim, T>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, m, s->nbr,
                          static_cast<const T*>(Vs), store ? static_cast<T*>(s->grad) : nullptr,
                          store ? static_cast<T*>(s->phi) : nullptr, static_cast<T*>(s->fv), per);
@@ -2145,6 +2149,7 @@ ma_status ensure_stage0_prims(ma_solver* s) {
 // plain launches.  A graph starts from valid stage-0 primitives (made here
 // when needed), so it never contains k_prim and leaves them valid.
 ma_status advance(ma_solver* s, int nsteps) {
+  if (nsteps > 0 && s->opt.second_order_space && !s->opt.viscous) s->grads_stale = true;
   if (s->graph_steps > 0) {
     while (nsteps >= s->graph_steps) {
       ma_status st0 = ensure_stage0_prims(s);
@@ -2443,9 +2448,42 @@ extern "C" ma_status ma_solver_get_residual(ma_solver* s, double* R) {
   return download_aos(s, s->R, 5, R);
 }
 
+namespace {
+
+template <class Lim, class T>
+void stage3_gradients(ma_solver* s) {
+  const int nb = nblocks(s->m.nc);
+  const int per = (nb + kXcds - 1) / kXcds;
+  hipLaunchKernelGGL((k_grad<true, true, Lim, T>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, s->m, s->nbr,
+                     static_cast<const T*>(s->V2), static_cast<T*>(s->grad), static_cast<T*>(s->phi),
+                     static_cast<T*>(s->fv), per);
+}
+
+// Inviscid second order stores no gradients or limiters while stepping.  The
+// last stage's primitives stay in V2 until the next step starts, so the same
+// kernel run on them gives the values that stage computed (and rewrites the
+// face values with the same bits).
+ma_status fill_gradients(ma_solver* s) {
+  if (!s->grads_stale) return MA_OK;
+  const bool va = s->opt.limiter == MA_LIMITER_VANALBADA;
+  if (s->fp32) {
+    if (va) stage3_gradients<ma::VanAlbadaLimiter, float>(s);
+    else stage3_gradients<ma::VenkatLimiter, float>(s);
+  } else {
+    if (va) stage3_gradients<ma::VanAlbadaLimiter, double>(s);
+    else stage3_gradients<ma::VenkatLimiter, double>(s);
+  }
+  MA_HIP(hipGetLastError());
+  s->grads_stale = false;
+  return MA_OK;
+}
+
+}  // namespace
+
 extern "C" ma_status ma_solver_get_gradients(ma_solver* s, double* grad, double* lim) {
   if (!s) return ma::fail(MA_ERR_INVALID, "null solver");
-  ma_status st = MA_OK;
+  ma_status st = fill_gradients(s);
+  if (st) return st;
   if (s->fp32) {
     if (grad && (st = download_aos(s, static_cast<const float*>(s->grad), 15, grad))) return st;
     if (lim && (st = download_aos(s, static_cast<const float*>(s->phi), 5, lim))) return st;
@@ -2875,7 +2913,7 @@ void multi_step_t(ma::Multi* M, int parity) {
     if (SO || Visc::isViscous)
       hipLaunchKernelGGL((k_grad_multi<SO, SO && !Visc::isViscous, Lim, T>), dim3(per_cells * kXcds),
                          dim3(kBlock), 0, M->stream, M->d_views, sc, n, per_cells, irk & 1,
-                         (Visc::isViscous || irk == 3) ? 1 : 0);
+                         Visc::isViscous ? 1 : 0);
     if (M->totals[1] > 0)
       hipLaunchKernelGGL(k_bc_multi<T>, dim3(M->totals[1]), dim3(kBlock), 0, M->stream, M->d_views, sb, n,
                          prandtl, irk & 1);
@@ -2923,6 +2961,7 @@ void solver_set_progress(ma_solver* s, int steps) {
   s->cur ^= (steps & 1);
   s->steps_done += steps;
   s->v0_valid = true;  // the multi-case flux kernel's last stage wrote the stage-0 primitives
+  if (steps > 0 && s->opt.second_order_space && !s->opt.viscous) s->grads_stale = true;
 }
 
 int solver_cur(const ma_solver* s) { return s->cur; }
