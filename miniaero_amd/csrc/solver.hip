@@ -2228,7 +2228,7 @@ ma_status create_run(int ncases, const ma_mesh_data* const* mds, const ma_option
 // its initial state; ms per step per million cells.
 ma_status time_candidate(int n, const ma_mesh_data* const* mds, const ma_options* so, const Tuning& c,
                          long long cells, double* v, bool* lattice) {
-  constexpr int kWarm = 2, kTimed = 4;
+  constexpr int kWarm = 2, kTimed = 8;
   ma_solver* t = nullptr;
   ma_status st = create_run(n, mds, so, &t, &c);
   if (st) return st;
@@ -2252,8 +2252,9 @@ ma_status time_candidate(int n, const ma_mesh_data* const* mds, const ma_options
   return MA_OK;
 }
 
-// Autotuner: builds the leading cases (at least 2^21 cells, or all of them)
-// with every candidate tile shape, then the best shape with several
+// Autotuner: builds a sample of the cases (every stride-th case, at least
+// 2^21 cells, or all of them; spread over the batch so that it sees the
+// batch's mix of case sizes) with every candidate tile shape, then the best shape with several
 // resident-workgroup counts, times a few steps of each from the initial
 // state with HIP events, and returns the fastest choice and its time in ms
 // per step per million cells.  Results do not depend on these choices
@@ -2265,17 +2266,25 @@ ma_status autotune(int ncases, const ma_mesh_data* const* mds, const ma_options*
                                    {{2, 8, 16}}, {{8, 2, 16}}, {{4, 16, 4}}, {{2, 16, 8}}, {{4, 4, 8}},
                                    {{0, 0, 0}}};
   static const int wgs[] = {1, 2, 3, 4};
-  int n = 0;
+  long long total = 0;
+  for (int k = 0; k < ncases; ++k) total += mds[k]->num_owned_cells;
+  const int stride = (int)std::max<long long>(1, std::min<long long>(ncases, total >> 21));
+  std::vector<const ma_mesh_data*> smd;
+  std::vector<ma_options> so;
   long long cells = 0;
-  while (n < ncases && cells < (1ll << 21)) cells += mds[n++]->num_owned_cells;
-  std::vector<ma_options> so(opts, opts + n);
+  for (int k = 0; k < ncases && cells < (1ll << 21); k += stride) {
+    smd.push_back(mds[k]);
+    so.push_back(opts[k]);
+    cells += mds[k]->num_owned_cells;
+  }
+  const int n = (int)smd.size();
   for (auto& o : so) o.autotune = 0;
   *best_ms = 0.0;
   *best = Tuning{kDefaultShape, 0};
   double best_v = 1e300;
   auto consider = [&](const Tuning& c, bool* lattice) -> ma_status {
     double v = 0.0;
-    ma_status st = time_candidate(n, mds, so.data(), c, cells, &v, lattice);
+    ma_status st = time_candidate(n, smd.data(), so.data(), c, cells, &v, lattice);
     if (st == MA_ERR_UNSUPPORTED) return MA_OK;  // e.g. a tile with too many faces for LDS
     if (st) return st;
     if (v < best_v) {
