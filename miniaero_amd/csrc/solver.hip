@@ -289,6 +289,15 @@ __global__ __launch_bounds__(kBlock) void k_prim(int nc, const double* __restric
   if (c < nc) prim_cell(nc, Un, R, alpha, V, c);
 }
 
+// Displacement x_f - x_c of the face in slot entry `ent` (side ent & 1), from
+// the FP64 centroids rounded to T; an empty slot reads face 0.
+template <class T>
+__device__ __forceinline__ void load_disp(const DevMesh& m, int ent, T* d) {
+  const int e = ent < 0 ? 0 : ent;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) d[k] = static_cast<const T*>(m.fdx)[(size_t)((e & 1) * 3 + k) * m.nf + (e >> 1)];
+}
+
 // Green-Gauss gradient (GreenGauss.h:51-270), then, for second order, the
 // stencil min/max (StencilLimiter.h:56-282) and the limiter
 // (StencilLimiter.h:288-500) of one cell, all from the cell's slot list.
@@ -308,15 +317,6 @@ __global__ __launch_bounds__(kBlock) void k_prim(int nc, const double* __restric
 // expression in the same order as the flux kernel's otherwise -- is stored
 // in fv[kMaxSlots * 5][nc], so the flux kernel gathers 5 values per face
 // side instead of 25; grad and phi are then stored only when non-null.
-// Displacement x_f - x_c of the face in slot entry `ent` (side ent & 1), from
-// the FP64 centroids rounded to T; an empty slot reads face 0.
-template <class T>
-__device__ __forceinline__ void load_disp(const DevMesh& m, int ent, T* d) {
-  const int e = ent < 0 ? 0 : ent;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) d[k] = static_cast<const T*>(m.fdx)[(size_t)((e & 1) * 3 + k) * m.nf + (e >> 1)];
-}
-
 template <bool kSecondOrder, bool kFaceVals, class Limiter, class T>
 __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restrict__ nbr, const T* __restrict__ V,
                                           T* __restrict__ grad, T* __restrict__ phi, T* __restrict__ fv, int c) {
@@ -2254,10 +2254,10 @@ ma_status time_candidate(int n, const ma_mesh_data* const* mds, const ma_options
 
 // Autotuner: builds a sample of the cases (every stride-th case, at least
 // 2^21 cells, or all of them; spread over the batch so that it sees the
-// batch's mix of case sizes) with every candidate tile shape, then the best shape with several
-// resident-workgroup counts, times a few steps of each from the initial
-// state with HIP events, and returns the fastest choice and its time in ms
-// per step per million cells.  Results do not depend on these choices
+// batch's mix of case sizes) with every candidate tile shape, then the best
+// shape with several resident-workgroup counts, times a few steps of each
+// from the initial state with HIP events, and returns the fastest choice and
+// its time in ms per step per million cells.  Results do not depend on these choices
 // (every cell sums its faces in the same order whatever the tiles and
 // however the workgroups walk them), only the speed does.
 ma_status autotune(int ncases, const ma_mesh_data* const* mds, const ma_options* opts, Tuning* best,
