@@ -176,7 +176,9 @@ ma_status ma_solver_tuning(ma_solver* s, int* tile_shape, int* workgroups_per_cu
 ma_status ma_solver_get_state(ma_solver* s, double* U);
 /* Residual of the last stage, num_owned_cells x 5. */
 ma_status ma_solver_get_residual(ma_solver* s, double* R);
-/* Gradients (n x 5 x 3) and limiters (n x 5) of the last stage; either may be NULL. */
+/* Gradients (n x 5 x 3) and limiters (n x 5) of the last stage; either may be NULL.
+   Inviscid second order does not keep them while stepping: the call computes
+   them from the last stage's primitives (same values) on the solver's stream. */
 ma_status ma_solver_get_gradients(ma_solver* s, double* grad, double* lim);
 /* Simulated time and the dt of the last step. */
 ma_status ma_solver_get_time(ma_solver* s, double* time, double* last_dt);
