@@ -166,6 +166,38 @@ def test_gradients_limiters_and_residual_match():
         p.close()
 
 
+def test_face_value_gradients_on_demand():
+    """Inviscid second order keeps no gradients or limiters while stepping;
+    the getter computes the last stage's.  They equal the oracle's bit for
+    bit after graph replays, on a second call, after set_state (which keeps
+    them, as the oracle does) and after a further plain step."""
+    kw = dict(CONFIG_CASES["sod_vanalbada"], ntimesteps=5)
+    p = Pair(kw)
+    try:
+        def same_gradients(what):
+            g, lim = p.gpu.gradients()
+            og, olim = p.cpu.gradients()
+            _log(f"  {what}: gradients bitwise={np.array_equal(g, og)}, limiters bitwise={np.array_equal(lim, olim)}")
+            assert np.array_equal(g, og) and np.array_equal(lim, olim), what
+            return g, lim
+        p.gpu.set_graph(2)
+        G, R = p.step(4)
+        assert np.array_equal(G, R)
+        g1, l1 = same_gradients("after 4 steps as 2 graph replays")
+        g2, l2 = same_gradients("second call")
+        assert np.array_equal(g1, g2) and np.array_equal(l1, l2)
+        U = p.gpu.state()
+        p.gpu.set_state(U)
+        p.cpu.set_state(U)
+        same_gradients("after set_state")
+        p.gpu.set_graph(0)
+        G, R = p.step(1)
+        assert np.array_equal(G, R)
+        same_gradients("after one more plain step")
+    finally:
+        p.close()
+
+
 @pytest.mark.parametrize("name", ["plate_cfl_stretch", "sod_vanalbada", "perturbed_first_order_viscous"])
 def test_extreme_exponent_gradients_match(name):
     """Momenta near the bottom of the FP64 range (1e-306, and subnormal) in
