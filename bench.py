@@ -41,9 +41,9 @@ def algorithmic_bytes(kernel: str, nc: int, nif: int, nbf: int, second_order: bo
     RK accumulator and the residual are always FP64 (40 B per cell each);
     connectivity is 8 B per face.  face_values: the inviscid second-order
     scheme, where the gradient kernel writes each cell's extrapolated value
-    at each of its 6 faces (30 values per cell; gradients and limiters only
-    on the last of the 4 stages) and the flux kernel reads 5 of them per face
-    side instead of the cells' primitives, gradients and limiters."""
+    at each of its 6 faces (30 values per cell; gradients and limiters are not
+    stored while stepping) and the flux kernel reads 5 of them per face side
+    instead of the cells' primitives, gradients and limiters."""
     F = nif + nbf
     if kernel == "k_flux":
         # V, grad, phi, x_c, vol, U_acc read and U_acc written per cell;
@@ -62,7 +62,7 @@ def algorithmic_bytes(kernel: str, nc: int, nif: int, nbf: int, second_order: bo
         # V, vol, x_c read; grad (+ phi) written; left/right, area, x_f per face
         vals = 25 if kernel == "k_grad_limiter" else 20
         if face_values:
-            vals = 5 + 30 + 5  # V read; face values written; grad + phi on 1 stage in 4
+            vals = 5 + 30  # V read; face values written
         return (vals * t + 4 * g) * nc + (8 + 6 * g) * F
     if kernel == "k_prim":
         return (80.0 + 5 * t) * nc          # U_n, R read; V written
