@@ -203,13 +203,16 @@ struct IcParams {
   int cell0;         // first cell of the case in the caller's union numbering
 };
 
+// SplitMix64 (Steele, Lea and Flood, OOPSLA 2014): output number i for seed
+// s is mix64(s + i * 0x9E3779B97F4A7C15).
 __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   return z ^ (z >> 31);
 }
 
-// Counter-based uniform draw in [-1, 1) keyed by (seed, cell, k).
+// Counter-based uniform draw in [-1, 1) keyed by (seed, cell, k): SplitMix64
+// output number 8 cell + k + 1, top 53 bits.
 __device__ __forceinline__ double unif(unsigned long long seed, long long cell, int k) {
   unsigned long long z = mix64(seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(cell * 8 + k + 1));
   return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;

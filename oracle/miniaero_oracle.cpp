@@ -29,7 +29,10 @@
  *     uniform flow preserved exactly by the full RK4 step).
  * Config-mode extensions (stretching, per-side BCs, Van Albada in the stencil
  * limiter, CFL dt, custom ICs, Prandtl) have no reference counterpart and are
- * pinned only by closed forms and invariants.
+ * pinned by closed forms and invariants: tests/test_oracle_extensions.py
+ * (stretched nodes, Sod-type and perturbed ICs, the generator against the
+ * published SplitMix64 sequence, the Mach/Reynolds freestream) and
+ * tests/test_oracle.py (CFL time step, Van Albada).
  */
 #include "oracle.h"
 
@@ -356,6 +359,10 @@ extern "C" int orc_mesh_counts(const orc_mesh* m, int* c, int max) {
   return w;
 }
 
+extern "C" void orc_mesh_nodes(const orc_mesh* m, double* xyz) {
+  std::copy(m->node_xyz.begin(), m->node_xyz.end(), xyz);
+}
+
 extern "C" void orc_mesh_cells(const orc_mesh* m, double* xyz, double* vol) {
   if (xyz) std::copy(m->cell_xyz.begin(), m->cell_xyz.end(), xyz);
   if (vol) std::copy(m->cell_vol.begin(), m->cell_vol.end(), vol);
@@ -588,12 +595,15 @@ extern "C" double orc_vanalbada(double dumax, double dumin, double du) {
 /* ------------------------------------------------------------ solver -- */
 
 namespace {
+// SplitMix64 (Steele, Lea and Flood, OOPSLA 2014): output number i for seed s
+// is mix64(s + i * 0x9E3779B97F4A7C15).
 uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   return z ^ (z >> 31);
 }
-// Counter-based uniform draw in [-1, 1) for (seed, cell, k).
+// Counter-based uniform draw in [-1, 1) for (seed, cell, k): SplitMix64
+// output number 8 cell + k + 1, top 53 bits.
 double unif(uint64_t seed, long long cell, int k) {
   uint64_t z = mix64(seed + 0x9E3779B97F4A7C15ull * (uint64_t)(cell * 8 + k + 1));
   return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;

@@ -53,6 +53,9 @@ void orc_mesh_free(orc_mesh* m);
  * counts[3 + g] = faces in group g.  Returns the number of ints written. */
 int orc_mesh_counts(const orc_mesh* m, int* counts, int max);
 void orc_mesh_cells(const orc_mesh* m, double* xyz, double* vol);
+/* Node coordinates ((nx+1)(ny+1)(nz+1) x 3), node (i, j, k) at row
+ * (i (ny+1) + j)(nz+1) + k (Parallel3DMesh.h:453-460, one rank). */
+void orc_mesh_nodes(const orc_mesh* m, double* xyz);
 /* group = -1 for internal faces.  Any pointer may be NULL. kind gets the BC kind
  * (1..4) of the group, 0 for internal faces. */
 void orc_mesh_faces(const orc_mesh* m, int group, double* xf, double* a, double* t,

@@ -92,6 +92,7 @@ def lib():
         L.orc_mesh_free.argtypes = [P]
         L.orc_mesh_counts.argtypes = [P, _I, ctypes.c_int]
         L.orc_mesh_cells.argtypes = [P, _D, _D]
+        L.orc_mesh_nodes.argtypes = [P, _D]
         L.orc_mesh_faces.argtypes = [P, ctypes.c_int, _D, _D, _D, _D, _I, _I, _I, _I]
         L.orc_solver_new.restype = P
         L.orc_solver_new.argtypes = [P, ctypes.POINTER(orc_options)]
@@ -159,6 +160,13 @@ class OracleMesh:
         L.orc_mesh_cells(self.h, _dp(self.cell_xyz), _dp(self.cell_vol))
         self.internal = self._faces(-1, self.n_internal)
         self.groups = [self._faces(g, nf) for g, nf in enumerate(self.group_sizes)]
+
+    def nodes(self):
+        """Node coordinates as an (nx+1, ny+1, nz+1, 3) array indexed by (i, j, k)."""
+        o = self.opts
+        X = np.zeros(((o.nx + 1) * (o.ny + 1) * (o.nz + 1), 3))
+        lib().orc_mesh_nodes(self.h, _dp(X))
+        return X.reshape(o.nx + 1, o.ny + 1, o.nz + 1, 3)
 
     def _faces(self, g, nf):
         d = _faces_dict(nf)
