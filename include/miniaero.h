@@ -165,7 +165,9 @@ ma_status ma_solver_synchronize(ma_solver* s);
  * ma_solver_step runs whole groups of steps_per_graph steps as one graph
  * launch (captured on first use) and the remainder as plain launches; 0 (the
  * default) launches every kernel separately.  Same results either way.
- * No reference counterpart. */
+ * Graphs are captured and replayed only on the thread that created the
+ * solver: a step call from another thread that would use one fails with
+ * MA_ERR_UNSUPPORTED and runs nothing.  No reference counterpart. */
 ma_status ma_solver_set_graph(ma_solver* s, int steps_per_graph);
 /* Launch shape in use: tile shape (-1 -1 -1 for consecutive cells),
  * resident workgroups per CU (0 = one per tile) and, after autotuning, the

@@ -17,9 +17,6 @@ void destroy_plan(PlannedCase* pc);
 // Builds the planned case inside [region, region + region_bytes) on its own
 // stream (no device allocation); the plan's arrays are consumed.
 ma_status create_from_plan(PlannedCase* pc, void* region, size_t region_bytes, ma_solver** out);
-// ma_solver_step without the argument checks.
-ma_status solver_advance(ma_solver* s, int nsteps);
-hipStream_t solver_stream(const ma_solver* s);
 // Which of the case's two state buffers holds sol_n (0 or 1).
 int solver_cur(const ma_solver* s);
 // Records that `steps` steps ran outside the solver's own launches (in a

@@ -1110,6 +1110,8 @@ struct ma_solver {
   // sol_n again after a replay), one per starting value of cur.
   int graph_steps = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // thread that created the solver: the only one that captures or replays its graphs
+  std::thread::id owner = std::this_thread::get_id();
   TileShape shape = kDefaultShape;  // flux tile shape in use
   int wg_per_cu = 0;                 // resident workgroups per CU of k_flux (0: full grid)
   int cus = 256;                     // compute units of the device
@@ -2152,6 +2154,10 @@ ma_status ensure_stage0_prims(ma_solver* s) {
 // plain launches.  A graph starts from valid stage-0 primitives (made here
 // when needed), so it never contains k_prim and leaves them valid.
 ma_status advance(ma_solver* s, int nsteps) {
+  if (s->graph_steps > 0 && nsteps >= s->graph_steps && std::this_thread::get_id() != s->owner)
+    return ma::fail(MA_ERR_UNSUPPORTED,
+                    "graph steps are captured and replayed only on the thread that created the solver; "
+                    "call ma_solver_set_graph(s, 0) to step from another thread");
   if (nsteps > 0 && s->opt.second_order_space && !s->opt.viscous) s->grads_stale = true;
   if (s->graph_steps > 0) {
     while (nsteps >= s->graph_steps) {
@@ -2769,15 +2775,6 @@ ma_status create_from_plan(PlannedCase* pc, void* region, size_t region_bytes, m
   *out = s;
   return MA_OK;
 }
-
-ma_status solver_advance(ma_solver* s, int nsteps) {
-  ma_status st = advance(s, nsteps);
-  if (st) return st;
-  MA_HIP(hipGetLastError());
-  return MA_OK;
-}
-
-hipStream_t solver_stream(const ma_solver* s) { return s->stream; }
 
 }  // namespace ma
 

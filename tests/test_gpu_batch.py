@@ -153,6 +153,44 @@ def test_graph_replays_match_plain_launches(graph_steps, scheme):
         b.close()
 
 
+def test_graph_steps_only_on_the_creating_thread():
+    """Graph capture and replay are confined to the thread that created the
+    solver: from another thread a step that would use a graph is refused
+    with MA_ERR_UNSUPPORTED before anything is queued, shorter steps run as
+    plain launches, and the run continues on the creating thread with the
+    bits of an all-plain run."""
+    import threading
+
+    cases = [make_case(VISCOUS_CASES[1])]
+    a = M.BatchSolver(cases)
+    b = M.BatchSolver(cases)
+    out = {}
+
+    def other(n):
+        try:
+            b.step(n)
+            out[n] = "ok"
+        except M.MiniAeroError as e:
+            out[n] = e.status
+
+    try:
+        a.initialize()
+        b.initialize()
+        b.set_graph(2)
+        for n in (2, 1):
+            t = threading.Thread(target=other, args=(n,))
+            t.start()
+            t.join()
+        print(f"[gpu-batch] off-thread steps: {out}", flush=True)
+        assert out == {2: 2, 1: "ok"}
+        b.step(4)          # two graph replays on the creating thread
+        a.step(5)
+        assert np.array_equal(a.state(), b.state())
+    finally:
+        a.close()
+        b.close()
+
+
 @pytest.mark.parametrize("shape,wg", [((-1, 0, 0), 0), ((1, 1, 1), 0), ((2, 8, 16), 1), ((8, 8, 4), 0),
                                       ((16, 4, 4), 2), ((3, 5, 7), 0), ((4, 8, 8), 1), ((4, 8, 8), 3)])
 def test_tile_shape_does_not_change_results(shape, wg):
