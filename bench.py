@@ -1,7 +1,8 @@
 """Benchmark: RK4 cell-stage updates per second of the FP64 viscous
-second-order flat plate (256x128x64 cells, 2.1M) on one MI355X.
+second-order flat plate (256x128x64 cells, 2.1M) on one MI355X, with the
+other headline figures of the path in the same JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload flat_plate_c2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload suite]
 
 A step is one RK4 time step (4 stages) over the whole mesh, inputs resident
 in HBM.  value = cells * 4 * K / (wall time of K steps), timed between two
@@ -10,12 +11,19 @@ stream synchronisations.  The JSON line also carries:
                 its mean launch time (HIP events on the solver's stream),
                 against the 8 TB/s HBM peak; traffic = FETCH+WRITE bytes per
                 launch from the committed rocprofv3 counter summary of this
-                workload, when present;
+                workload, when present; path = the whole step's algorithmic
+                bytes over its wall time;
   cpu_baseline  the CPU oracle (a single-threaded port of the reference's
                 serial arithmetic) timed on a bounded sample of the same
                 physics on this host's cores: one oracle process per core
                 (at most 16), all running the sample at once; the value is
-                their aggregate rate.
+                their aggregate rate;
+  workloads     (default suite) the FP64 path on the 512x256x64 plate (8.4M
+                cells) with its roofline, the precision form on that mesh
+                (500 steps, per-variable error against FP64), the 512-case
+                batched form with its per-case latency, the first 200
+                requests of the served stream with p50/p99 latency, and the
+                3D Sod case run in full on one CPU core and on the GPU.
 """
 from __future__ import annotations
 
@@ -71,19 +79,33 @@ def algorithmic_bytes(kernel: str, nc: int, nif: int, nbf: int, second_order: bo
     return 0.0
 
 
-def path_bytes_per_step(nc: int, nif: int, nbf: int, second_order: bool, viscous: bool) -> float:
+def path_bytes_per_step(nc: int, nif: int, nbf: int, second_order: bool, viscous: bool,
+                        fp32: bool = False) -> float:
     """Compulsory HBM bytes of one RK4 step of the whole path (4 stages):
     the per-stage figures of the fused target design -- 824 N_c + 168 F
     (second order, inviscid or viscous), 536 N_c + 112 F (first-order
-    viscous), 208 N_c + 80 F (first-order inviscid); FP64 sizes."""
+    viscous), 208 N_c + 80 F (first-order inviscid); FP64 sizes.  fp32: the
+    precision form's second-order figure with gradients, limiters and
+    geometry at 4 bytes and the state, accumulator and residual at 8 --
+    572 N_c + 96 F (pass A 184 N_c + 20 F, B 132 N_c + 20 F, C 256 N_c + 56 F)."""
     F = nif + nbf
-    if second_order:
+    if second_order and fp32:
+        per = 572.0 * nc + 96.0 * F
+    elif second_order:
         per = 824.0 * nc + 168.0 * F
     elif viscous:
         per = 536.0 * nc + 112.0 * F
     else:
         per = 208.0 * nc + 80.0 * F
     return 4.0 * per
+
+
+def box_faces(nx: int, ny: int, nz: int):
+    """Cells, internal faces and boundary faces of an nx x ny x nz box mesh."""
+    nc = nx * ny * nz
+    nif = (nx - 1) * ny * nz + nx * (ny - 1) * nz + nx * ny * (nz - 1)
+    nbf = 2 * (ny * nz + nx * nz + nx * ny)
+    return nc, nif, nbf
 
 
 def _oracle_run(cases, steps: int):
@@ -165,10 +187,9 @@ def cpu_baseline(opts) -> dict:
                       f"{steps} RK4 steps, same physics/options, {secs:.1f} s"}
 
 
-def cpu_baseline_batch(cases, min_cells: int = 600_000) -> dict:
+def cpu_baseline_batch(cases, min_cells: int = 600_000, steps: int = 3) -> dict:
     """The CPU oracle over the batch's first cases, one after another, until
-    they hold min_cells cells, 3 RK4 steps each."""
-    steps = 3
+    they hold min_cells cells, `steps` RK4 steps each."""
     chosen, n = [], 0
     for o in cases:
         chosen.append(o)
@@ -178,7 +199,7 @@ def cpu_baseline_batch(cases, min_cells: int = 600_000) -> dict:
     rate, cells, secs, n = _oracle_rate(chosen, steps)
     return {"value": rate, "unit": "cell-stage updates/s", "cores": n, "kind": "port",
             "sample": f"oracle (single-thread C++ restatement of the reference's serial path), one process "
-                      f"per core on {n} cores, each running the first {len(chosen)} cases of the batch "
+                      f"per core on {n} cores, each running the first {len(chosen)} cases of the sample "
                       f"({cells} cells), {steps} RK4 steps each, {secs:.1f} s"}
 
 
@@ -278,8 +299,8 @@ def bench_single(args, M, opts, copy_gbs) -> dict:
     roof = roofline(solver, args.profile_steps, nc, nif, nbf, so, opts.precision != 0,
                     lambda k: load_traffic(args.workload, k), copy_gbs,
                     face_values=so and not opts.viscous)
-    if opts.precision == 0:
-        pb = path_bytes_per_step(nc, nif, nbf, so, bool(opts.viscous))
+    if opts.precision == 0 or so:
+        pb = path_bytes_per_step(nc, nif, nbf, so, bool(opts.viscous), fp32=opts.precision != 0)
         path_gbs = pb / (elapsed / args.steps) / 1e9
         roof["path"] = {"algorithmic_bytes_per_step": pb, "achieved_GBps": round(path_gbs, 1),
                         "frac": round(path_gbs / HBM_PEAK_GBS, 4)}
@@ -344,6 +365,10 @@ def bench_batch(args, M, cases, copy_gbs) -> dict:
     roof = roofline(solver, args.profile_steps, nc, nif, nbf, True, fp32,
                     lambda k: load_traffic(args.workload, k), copy_gbs,
                     face_values=bool(cases[0].second_order_space) and not cases[0].viscous)
+    pb = path_bytes_per_step(nc, nif, nbf, True, bool(cases[0].viscous), fp32=fp32)
+    path_gbs = pb / (elapsed / args.steps) / 1e9
+    roof["path"] = {"algorithmic_bytes_per_step": pb, "achieved_GBps": round(path_gbs, 1),
+                    "frac": round(path_gbs / HBM_PEAK_GBS, 4)}
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -450,6 +475,16 @@ def bench_served(args, M, stream, copy_gbs) -> dict:
                           "momentum components scaled by the momentum-vector norm",
                 **{k: max(e[k] for e in errs) for k in errs[0]}}
     cells = np.array([r["cells"] for r in rs])
+    path_bytes = 0.0
+    for r in rs:
+        o = opts_of[r["id"]]
+        nc, nif, nbf = box_faces(o.nx, o.ny, o.nz)
+        path_bytes += path_bytes_per_step(nc, nif, nbf, True, bool(o.viscous), fp32=o.precision != 0) * r["steps"]
+    path_gbs = path_bytes / span / 1e9
+    roof = {"bound": "hbm", "achieved": round(path_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(path_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "path: every kernel of every case, algorithmic bytes of the precision form over "
+                      "the span from the first submission to the last completion"}
     config = {"workload": args.workload, "requests": len(rs),
               "arrivals": "Poisson, 200 per second (seeded)", "cells_per_case":
                   {"min": int(cells.min()), "mean": float(cells.mean()), "max": int(cells.max())},
@@ -462,8 +497,140 @@ def bench_served(args, M, stream, copy_gbs) -> dict:
                                  "max": float(lat.max())},
               "admission_wait_s": {"p50": float(np.percentile(wait, 50)), "p99": float(np.percentile(wait, 99))},
               "span_s": span, "offered_cell_stages_per_s": work / max(stream[-1][0] - stream[0][0], 1e-9)}
-    return dict(value=work / span, elapsed=span, config=config, roofline=None, cpu_baseline=None, fp32=True,
+    return dict(value=work / span, elapsed=span, config=config, roofline=roof, cpu_baseline=None, fp32=True,
                 extra=extra, steps=None)
+
+
+def _sub(args, **over):
+    a = argparse.Namespace(**vars(args))
+    for k, v in over.items():
+        setattr(a, k, v)
+    return a
+
+
+def _compact(r, steps) -> dict:
+    """The fields of one workload's result that go into the suite line."""
+    out = {"value": r["value"], "unit": "cell-stage updates/s"}
+    if r.get("steps", 1) is not None:
+        out["ms_per_step"] = r["elapsed"] / steps * 1e3
+        out["steps"] = steps
+    roof = r.get("roofline")
+    if roof:
+        out["roofline"] = {k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic", "kernel",
+                                                "mean_launch_ms", "path") if k in roof}
+    out["cpu_baseline"] = r.get("cpu_baseline")
+    out["dtype"] = "f32 arithmetic, f64 state" if r["fp32"] else "f64"
+    out["config"] = r["config"]
+    out.update(r.get("extra", {}))
+    return out
+
+
+def sod_c1_full(M) -> dict:
+    """The 3D Sod case (64x16x16, first order, 100 steps) run in full on the
+    GPU and in full on one host core by the CPU oracle; the two final states
+    are compared bit for bit."""
+    import dataclasses
+
+    import numpy as np
+
+    from miniaero_amd.configs import sod_c1
+    from oracle import oracle as O
+
+    o = sod_c1()
+    md = M.Parallel3DMesh(o.nx, o.ny, o.nz, o.lx, o.ly, o.lz, o.problem_type, o.angle, options=o).fillMeshData()
+    s = M.TimeSolverExplicitRK4(md, o)
+    s.initialize()
+    s.step(2)
+    s.synchronize()
+    s.initialize()
+    t0 = time.perf_counter()
+    s.step(o.ntimesteps)
+    s.synchronize()
+    gpu_s = time.perf_counter() - t0
+    U = s.state()
+    s.close()
+    kw = {f.name: getattr(o, f.name) for f in dataclasses.fields(o)
+          if f.name not in ("output_results", "output_frequency")}
+    om = O.OracleMesh(O.make_options(**kw))
+    os_ = O.OracleSolver(om)
+    os_.init()
+    t0 = time.perf_counter()
+    os_.step(o.ntimesteps)
+    cpu_s = time.perf_counter() - t0
+    work = om.ncells * 4 * o.ntimesteps
+    return {"value": work / gpu_s, "unit": "cell-stage updates/s", "ms_per_step": gpu_s / o.ntimesteps * 1e3,
+            "steps": o.ntimesteps, "dtype": "f64",
+            "note": "16k cells: the GPU time is launch latency, not bandwidth",
+            "matches_oracle_bitwise": bool(np.array_equal(U, os_.state())),
+            "cpu_baseline": {"value": work / cpu_s, "unit": "cell-stage updates/s", "cores": 1, "kind": "port",
+                             "sample": f"the whole case ({om.ncells} cells, {o.ntimesteps} RK4 steps) on one "
+                                       f"core by the oracle (single-thread C++ restatement of the reference's "
+                                       f"serial path), {cpu_s:.1f} s"},
+            "config": {"workload": "sod_c1", "mesh": f"{o.nx}x{o.ny}x{o.nz}", "cells": om.ncells,
+                       "physics": "inviscid Roe, 1st order, Extrapolate on every side", "dt": o.dt}}
+
+
+def bench_suite(args, M, copy_gbs) -> dict:
+    """The primary line (FP64 viscous plate, 2.1M cells) plus every other
+    headline figure of the path, each under workloads[name]."""
+    from miniaero_amd.configs import BATCH_WORKLOADS, SERVED_WORKLOADS, WORKLOADS
+
+    def note(msg):
+        print(f"[bench] suite: {msg}", file=sys.stderr, flush=True)
+
+    note("flat_plate_c2 (primary)")
+    r = bench_single(_sub(args, workload="flat_plate_c2", graph=0), M, WORKLOADS["flat_plate_c2"](), copy_gbs)
+    plain = r["value"]
+    wl = {}
+
+    def attempt(name, fn):
+        note(name)
+        t0 = time.perf_counter()
+        try:
+            wl[name] = fn()
+        except Exception as e:  # a failed leg is reported, the others still run
+            wl[name] = {"error": f"{type(e).__name__}: {e}"}
+        wl[name]["bench_seconds"] = round(time.perf_counter() - t0, 1)
+        if "value" in wl[name]:
+            wl[name]["vs_plain_fp64_c2"] = round(wl[name]["value"] / plain, 3)
+        note(f"{name} done in {wl[name]['bench_seconds']} s")
+
+    same_physics = {"see": "cpu_baseline of the primary line: the same physics, timed on the same host"}
+
+    def fp64_large():
+        a = _sub(args, workload="flat_plate_c3_fp64", graph=0, no_cpu_baseline=True, compare_fp64=False)
+        x = _compact(bench_single(a, M, WORKLOADS["flat_plate_c3"]().replace(precision=0), copy_gbs), a.steps)
+        x["cpu_baseline"] = same_physics
+        return x
+
+    def precision_c3():
+        o = WORKLOADS["flat_plate_c3"]()
+        a = _sub(args, workload="flat_plate_c3", graph=0, no_cpu_baseline=True, compare_fp64=True,
+                 warmup=min(args.warmup, 5), steps=o.ntimesteps - min(args.warmup, 5))
+        x = _compact(bench_single(a, M, o, copy_gbs), a.steps)
+        x["cpu_baseline"] = same_physics
+        return x
+
+    def batched_c4():
+        a = _sub(args, workload="batched_c4", graph=2, compare_fp64=False)
+        return _compact(bench_batch(a, M, BATCH_WORKLOADS["batched_c4"](), copy_gbs), a.steps)
+
+    def served_200():
+        a = _sub(args, workload="served_c5", cases=200, compare_fp64=False)
+        stream = SERVED_WORKLOADS["served_c5"](200)
+        x = _compact(bench_served(a, M, stream, copy_gbs), a.steps)
+        if not args.no_cpu_baseline:
+            small = [o for _, o in stream if o.nx * o.ny * o.nz <= 300_000]
+            x["cpu_baseline"] = cpu_baseline_batch(small, min_cells=300_000, steps=2)
+        return x
+
+    attempt("flat_plate_fp64_8M", fp64_large)
+    attempt("precision_c3", precision_c3)
+    attempt("batched_c4", batched_c4)
+    attempt("served_c5_first200", served_200)
+    attempt("sod_c1", lambda: sod_c1_full(M))
+    r["extra"] = dict(r.get("extra", {}), workloads=wl)
+    return r
 
 
 def main() -> int:
@@ -471,7 +638,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="flat_plate_c2")
+    ap.add_argument("--workload", default="suite",
+                    help="suite (default: the primary flat-plate line plus every other headline workload), "
+                         "or one of flat_plate_c2, flat_plate_c3, sod_c1, batched_c4, served_c5")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", type=int, default=None,
@@ -516,7 +685,9 @@ def main() -> int:
     if args.graph is None:
         args.graph = 2 if args.workload in BATCH_WORKLOADS else 0
     try:
-        if args.workload in SERVED_WORKLOADS:
+        if args.workload == "suite":
+            r = bench_suite(args, M, copy_gbs)
+        elif args.workload in SERVED_WORKLOADS:
             r = bench_served(args, M, SERVED_WORKLOADS[args.workload](args.cases), copy_gbs)
         elif args.workload in BATCH_WORKLOADS:
             cases = BATCH_WORKLOADS[args.workload]()
