@@ -7,6 +7,10 @@
 //   NewtonianViscousFlux / NoViscousFlux (isViscous)    Viscous_Flux.h:39-99
 //   VenkatLimiter::limit(dumax, dumin, du, deltax3)     VenkatLimiter.h:45
 //   VanAlbadaLimiter::limit(dumax, dumin, du)           VanAlbadaLimiter.h:45
+//   ExtrapolateBC, TangentBC, InflowBC, NoSlipBC        Extrapolate_BC.h:45-102,
+//                                                       Tangent_BC.h:47-123,
+//                                                       Inflow_BC.h:45-111,
+//                                                       NoSlip_BC.h:47-157
 //
 // Everything is templated on the arithmetic type T.  With T = double every
 // expression keeps the reference's operation order (left-to-right
@@ -18,6 +22,9 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
+#include <utility>
 
 #define MA_HD __host__ __device__ __forceinline__
 
@@ -60,9 +67,15 @@ MA_HD T smax(T a, T b) { return (a < b) ? b : a; }
 // normal, tangent and binormal (Roe_Flux.h:100-118).  It depends on the face
 // alone, so the solver computes it once per face at setup, with the same
 // operations, instead of once per flux evaluation.
+// A face as the flux policies see it: the Faces<Device> vectors as stored
+// (area-weighted normal n, tangent t, binormal b; Faces.h:42-71) and the
+// unit vectors Roe_Flux.h:100-118 makes of them.  Unused members cost
+// nothing: the kernels build the frame inline.
 template <class T>
 struct FaceFrameT {
   T n[3];   // area-weighted normal
+  T t[3];   // tangent as stored
+  T b[3];   // binormal as stored (n x t, length |n|)
   T nn;     // |n|
   T nu[3], tu[3], bu[3];
 };
@@ -76,6 +89,8 @@ MA_HD FaceFrameT<T> make_frame_t(const T* fn, const T* ft, const T* fb) {
   const T bn = sqrt(fb[0] * fb[0] + fb[1] * fb[1] + fb[2] * fb[2]);
   for (int d = 0; d < 3; ++d) {
     f.n[d] = fn[d];
+    f.t[d] = ft[d];
+    f.b[d] = fb[d];
     f.nu[d] = fn[d] / nn;
     f.tu[d] = ft[d] / tn;
     f.bu[d] = fb[d] / bn;
@@ -95,28 +110,25 @@ MA_HD FaceFrame make_frame(const double* fn, const double* ft, const double* fb)
 // t and b are exactly these, the frame can be rebuilt from a on the fly with
 // the same operations, which is cheaper than reading it.  In FP64 this gives
 // the reference's bits; the FP32-arithmetic form builds it in FP32.
+//
+// The three cases share one form: with i3 = i1 + 2 (mod 3), p = a[i1] and
+// q = a[i3], den = sqrt(p^2 + q^2), t[i1] = q / den, t[i3] = -p / den.  The
+// operands are selected first and the square root and the two divisions
+// computed once (a face-parallel wavefront mixes orientations, and a
+// branch per case would run all three).
 template <class T>
 MA_HD void tangent_binormal_t(const T* a, T* t, T* b) {
   const T m0 = fabs(a[0]), m1 = fabs(a[1]), m2 = fabs(a[2]);
   int i1 = 0;
   if (m1 > m0) i1 = 1;
   if (m2 > (i1 == 0 ? m0 : m1)) i1 = 2;
-  if (i1 == 0) {
-    const T den = sqrt(a[0] * a[0] + a[2] * a[2]);
-    t[0] = a[2] / den;
-    t[1] = T(0.0);
-    t[2] = -a[0] / den;
-  } else if (i1 == 1) {
-    const T den = sqrt(a[1] * a[1] + a[0] * a[0]);
-    t[0] = -a[1] / den;
-    t[1] = a[0] / den;
-    t[2] = T(0.0);
-  } else {
-    const T den = sqrt(a[2] * a[2] + a[1] * a[1]);
-    t[0] = T(0.0);
-    t[1] = -a[2] / den;
-    t[2] = a[1] / den;
-  }
+  const T p = i1 == 0 ? a[0] : (i1 == 1 ? a[1] : a[2]);
+  const T q = i1 == 0 ? a[2] : (i1 == 1 ? a[0] : a[1]);
+  const T den = sqrt(p * p + q * q);
+  const T tp = q / den, tq = -p / den;
+  t[0] = i1 == 0 ? tp : (i1 == 1 ? tq : T(0.0));
+  t[1] = i1 == 0 ? T(0.0) : (i1 == 1 ? tp : tq);
+  t[2] = i1 == 0 ? tq : (i1 == 1 ? T(0.0) : tp);
   b[0] = a[1] * t[2] - t[1] * a[2];
   b[1] = -a[0] * t[2] + t[0] * a[2];
   b[2] = a[0] * t[1] - t[0] * a[1];
@@ -133,12 +145,41 @@ MA_HD FaceFrameT<T> frame_from_area_t(const T* a) {
 
 MA_HD FaceFrame frame_from_area(const double* a) { return frame_from_area_t<double>(a); }
 
+// ---- inviscid flux policies ----
+// The reference templates compute_face_flux and every boundary functor on an
+// inviscid policy with
+//   void compute_flux(const double* V_L, const double* V_R, double* F,
+//                     const double* n, const double* t, const double* b) const
+// (Roe_Flux.h:50-54): primitives of both sides, the face's stored normal,
+// tangent and binormal.  The kernels here take any policy of that shape,
+// templated on the arithmetic type.  A policy may also offer
+//   void flux_frame(const T* V_L, const T* V_R, T* F, const FaceFrameT<T>&) const
+// which receives the unit frame already built (RoeFlux does: the kernels
+// rebuild the frame once per face and the policy would otherwise normalise
+// it again); inviscid_flux() calls whichever the policy has.
+template <class Inv, class T, class = void>
+struct takes_frame : std::false_type {};
+template <class Inv, class T>
+struct takes_frame<Inv, T,
+                   decltype(std::declval<const Inv&>().flux_frame(
+                                std::declval<const T*>(), std::declval<const T*>(), std::declval<T*>(),
+                                std::declval<const FaceFrameT<T>&>()),
+                            void())> : std::true_type {};
+
+template <class Inv, class T>
+MA_HD void inviscid_flux(const Inv& inv, const T* vl, const T* vr, T* F, const FaceFrameT<T>& fr) {
+  if constexpr (takes_frame<Inv, T>::value)
+    inv.flux_frame(vl, vr, F, fr);
+  else
+    inv.compute_flux(vl, vr, F, fr.n, fr.t, fr.b);
+}
+
 struct RoeFlux {
-  // Roe_Flux.h:50-54 signature: builds the face frame, then the flux.
-  MA_HD void compute_flux(const double* pl, const double* pr, double* flux, const double* fn,
-                          const double* ft, const double* fb) const {
-    const FaceFrame fr = make_frame(fn, ft, fb);
-    flux_frame(pl, pr, flux, fr);
+  // Roe_Flux.h:50-54 signature: builds the face frame (Roe_Flux.h:100-118),
+  // then the flux; the same bits as flux_frame on make_frame_t(n, t, b).
+  template <class T>
+  MA_HD void compute_flux(const T* pl, const T* pr, T* flux, const T* fn, const T* ft, const T* fb) const {
+    flux_frame(pl, pr, flux, make_frame_t<T>(fn, ft, fb));
   }
 
   // Roe flux with the eigenvalue fix of Roe_Flux.h:157-186.  The 5x5 left and
@@ -328,37 +369,126 @@ struct NoViscousFlux {
   MA_HD void compute_flux(const T[5][3], const T*, const T*, T*) const {}
 };
 
+// ---- boundary policies ----
+// The reference applies each boundary group with a functor
+// compute_<kind>BC_flux<Device, Inviscid[, Viscous]> whose operator()(i)
+// evaluates face i (Extrapolate_BC.h:60-101, Tangent_BC.h:62-122,
+// Inflow_BC.h:61-110, NoSlip_BC.h:65-156).  Here each kind is a policy whose
+// operator() evaluates one face from what the kernel loaded for it -- the
+// primitives V of the cell inside, the face frame, the conserved inflow
+// state (Inflow) and 1/|x_f - x_c| (NoSlip) -- with the inviscid policy and
+// the wall's viscous policy as arguments, and writes the face's contribution
+// to its cell's flux sum (the operand of the reference's atomic_add).
+
+// Tangent_BC.h:86-102 / NoSlip_BC.h:96-112: the mirror state, normal
+// velocity reflected; |n| of the frame is the reference's area_norm bit for bit.
+template <class T>
+MA_HD void mirror_state(const T* vc, const FaceFrameT<T>& fr, T* vr) {
+  const T* n = fr.n;
+  const T an = fr.nn;
+  T ub = 0.0;
+  ub += vc[1] * n[0] / an;
+  ub += vc[2] * n[1] / an;
+  ub += vc[3] * n[2] / an;
+  vr[0] = vc[0];
+  vr[1] = vc[1] - T(2) * ub * n[0] / an;
+  vr[2] = vc[2] - T(2) * ub * n[1] / an;
+  vr[3] = vc[3] - T(2) * ub * n[2] / an;
+  vr[4] = vc[4];
+}
+
+struct ExtrapolateBC {  // compute_extrapolateBC_flux: the cell's own state on both sides
+  template <class T, class Inv, class Wall>
+  MA_HD void operator()(const T* vc, const FaceFrameT<T>& fr, const double*, T, const Inv& inv, const Wall&,
+                        T* out) const {
+    T F[5];
+    inviscid_flux(inv, vc, vc, F, fr);
+    for (int k = 0; k < 5; ++k) out[k] = -F[k];
+  }
+};
+
+struct TangentBC {  // compute_tangentBC_flux: slip wall, mirror state outside
+  template <class T, class Inv, class Wall>
+  MA_HD void operator()(const T* vc, const FaceFrameT<T>& fr, const double*, T, const Inv& inv, const Wall&,
+                        T* out) const {
+    T vr[5], F[5];
+    mirror_state(vc, fr, vr);
+    inviscid_flux(inv, vc, vr, F, fr);
+    for (int k = 0; k < 5; ++k) out[k] = -F[k];
+  }
+};
+
+struct InflowBC {  // compute_inflowBC_flux: the inflow state outside
+  template <class T, class Inv, class Wall>
+  MA_HD void operator()(const T* vc, const FaceFrameT<T>& fr, const double* state, T, const Inv& inv,
+                        const Wall&, T* out) const {
+    double vi[5];
+    primitives(state, vi);
+    T vr[5], F[5];
+    for (int k = 0; k < 5; ++k) vr[k] = T(vi[k]);
+    inviscid_flux(inv, vc, vr, F, fr);
+    for (int k = 0; k < 5; ++k) out[k] = -F[k];
+  }
+};
+
+// compute_NoSlipBC_flux: the mirror state's inviscid flux, plus the viscous
+// flux of the one-sided gradient from the cell to the wall state
+// (rho, 0, 0, 0, T) over the centroid-to-face distance (NoSlip_BC.h:114-147).
+struct NoSlipBC {
+  template <class T, class Inv, class Wall>
+  MA_HD void operator()(const T* vc, const FaceFrameT<T>& fr, const double*, T wall_inv, const Inv& inv,
+                        const Wall& wallv, T* out) const {
+    T vr[5], F[5];
+    mirror_state(vc, fr, vr);
+    inviscid_flux(inv, vc, vr, F, fr);
+    const T vf[5] = {vc[0], T(0.0), T(0.0), T(0.0), vc[4]};
+    T gf[5][3], fv[5];
+    for (int k = 0; k < 5; ++k)
+      for (int d = 0; d < 3; ++d) gf[k][d] = (vf[k] - vc[k]) * fr.nu[d] * wall_inv;
+    wallv.compute_flux(gf, vf, fr.n, fv);
+    for (int k = 0; k < 5; ++k) out[k] = -F[k] + fv[k];
+  }
+};
+
+// The boundary policy of each kind the mesh's groups name (MA_BC_*).  A
+// caller's own boundary treatment is a struct of the same shape in place of
+// one of these.
+struct ReferenceBoundaries {
+  using Extrapolate = ExtrapolateBC;
+  using Tangent = TangentBC;
+  using NoSlip = NoSlipBC;
+  using Inflow = InflowBC;
+};
+
 struct VenkatLimiter {
   enum { kind = 0 };
-  // VenkatLimiter.h:45-73; deltax3 is the squared face distance (StencilLimiter.h:426-455).
+  // VenkatLimiter.h:45-73; deltax3 is the squared face distance
+  // (StencilLimiter.h:426-455).  The reference's two branches apply one
+  // formula to dumax (du > 1e-40) or dumin (du < -1e-40); the bound is
+  // selected first and the formula evaluated once, with the same operations,
+  // so a wavefront whose lanes take both branches does one division, not two.
   template <class T>
   static MA_HD T limit(T dumax, T dumin, T du, T deltax3) {
     T beta = 1;
     T eps2 = deltax3 * beta * beta * beta;
-    if (du > T(1e-40)) {
-      T num = (dumax * dumax + eps2) * du + T(2) * du * du * dumax;
-      T den = du * (dumax * dumax + T(2) * du * du + dumax * du + eps2);
-      return num / den;
-    } else if (du < -T(1e-40)) {
-      T num = (dumin * dumin + eps2) * du + T(2) * du * du * dumin;
-      T den = du * (dumin * dumin + T(2) * du * du + dumin * du + eps2);
-      return num / den;
-    }
-    return 1;
+    const T d = du > T(0) ? dumax : dumin;
+    T num = (d * d + eps2) * du + T(2) * du * du * d;
+    T den = du * (d * d + T(2) * du * du + d * du + eps2);
+    const T phi = num / den;
+    return (du > T(1e-40) || du < -T(1e-40)) ? phi : T(1);
   }
 };
 
 struct VanAlbadaLimiter {
   enum { kind = 1 };
   // VanAlbadaLimiter.h:45-65 (defined but never called by the reference).
+  // As in VenkatLimiter, the bound is selected before the one division.
   template <class T>
   static MA_HD T limit(T dumax, T dumin, T du, T /*deltax3*/) {
     const T eps = T(2.220446049250313080847e-16);  // DBL_EPSILON
-    T y = 2;
-    if (du > eps)
-      y = dumax / du;
-    else if (du < -eps)
-      y = dumin / du;
+    const T d = du > eps ? dumax : dumin;
+    const T q = d / du;
+    const T y = (du > eps || du < -eps) ? q : T(2);
     T phi = 1;
     if (y < T(2)) {
       phi = (T(4) * y - y * y) / (y * y - T(4) * y + T(8));

@@ -46,9 +46,9 @@
 //           xc[3][nc] for the initial condition only
 //   faces   n[3][nf] (area vector; an FP32 copy in the FP32 form),
 //           disp[6][nf] (x_f - x_c of the left and right cell, arithmetic
-//           type), left[nf], right[nf], kind[nf] (int8), wall[nf]; the unit
-//           frame (|n|, n, t, b) is rebuilt from n in the kernels, and stored
-//           as frame[10][nf] only for meshes whose tangents do not follow the
+//           type), left[nf], right[nf], kind[nf] (int8), wall[nf]; tangent
+//           and binormal are rebuilt from n in the kernels, and stored as
+//           frame[6][nf] only for meshes whose tangents do not follow the
 //           reference construction; internal faces in processing (left-cell)
 //           order, then the boundary groups
 //   tiles   ptr[ntiles+1], faces[] (internal faces touching the tile) with
@@ -95,8 +95,8 @@ struct DevMesh {
   const int* __restrict__ slot;    // kMaxSlots planes of nc; entry = face*2 + side, -1 empty
   const double* __restrict__ fn;   // 3 planes of nf: area vector
   const float* __restrict__ fn32;  // the same in FP32 (FP32-arithmetic form only)
-  const double* __restrict__ ffr;  // 10 planes of nf: |n|, unit n, unit t, unit b (or null)
-  int derive_frame;                // 1: rebuild the frame from fn (ffr unused)
+  const double* __restrict__ ffr;  // 6 planes of nf: tangent, binormal as given (or null)
+  int derive_frame;                // 1: rebuild tangent and binormal from fn (ffr unused)
   // 6 planes of nf in the arithmetic type: T(x_f - x_left) (planes 0-2) and
   // T(x_f - x_right) (3-5, internal faces), from FP64 centroids
   const void* __restrict__ fdx;
@@ -153,31 +153,16 @@ __device__ __forceinline__ const float* area<float>(const DevMesh& m) {
 // a, or read from the stored frame planes.
 template <class T>
 __device__ __forceinline__ ma::FaceFrameT<T> frame_of(const DevMesh& m, int f, const T* a) {
-  ma::FaceFrameT<T> fr;
-  if (m.derive_frame) {
-    const ma::FaceFrameT<T> d = ma::frame_from_area_t<T>(a);
-    fr.nn = d.nn;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      fr.n[k] = d.n[k];
-      fr.nu[k] = d.nu[k];
-      fr.tu[k] = d.tu[k];
-      fr.bu[k] = d.bu[k];
-    }
-    return fr;
-  }
-#pragma unroll
-  for (int d = 0; d < 3; ++d) fr.n[d] = a[d];
+  if (m.derive_frame) return ma::frame_from_area_t<T>(a);
   const double* p = m.ffr;
   const size_t nf = m.nf;
-  fr.nn = T(p[f]);
+  T t[3], b[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    fr.nu[d] = T(p[(1 + d) * nf + f]);
-    fr.tu[d] = T(p[(4 + d) * nf + f]);
-    fr.bu[d] = T(p[(7 + d) * nf + f]);
+    t[d] = T(p[d * nf + f]);
+    b[d] = T(p[(3 + d) * nf + f]);
   }
-  return fr;
+  return ma::make_frame_t<T>(a, t, b);
 }
 
 template <class T>
@@ -455,8 +440,15 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
   }
 }
 
+// Waves per SIMD the FP64 second-order gradient kernels are register-budgeted
+// for: the compiler's own choice can land just over the 168 registers that
+// allow 3 (then 2 waves run); other forms no constraint.
+template <bool kSecondOrder, class T>
+constexpr int kGradWaves = sizeof(T) == 8 && kSecondOrder ? 3 : 1;
+
 template <bool kSecondOrder, bool kFaceVals, class Limiter, class T>
-__global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const int* __restrict__ nbr, const T* __restrict__ V,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGradWaves<kSecondOrder, T>)))
+void k_grad(DevMesh m, const int* __restrict__ nbr, const T* __restrict__ V,
                                                  T* __restrict__ grad, T* __restrict__ phi, T* __restrict__ fv,
                                                  int per_xcd) {
   // XCD-aware: each XCD takes a contiguous run of blocks (see k_flux)
@@ -475,11 +467,12 @@ __global__ __launch_bounds__(kBlock) void k_grad(DevMesh m, const int* __restric
 // sL of cell L, slot sR of cell Rr), stored by the gradient kernel with the
 // same arithmetic.  The result is a pure function of (f, V, grad, phi), so
 // every workgroup that evaluates face f gets the same bits.
-template <bool kSecondOrder, class Visc, class T>
+template <bool kSecondOrder, class Inv, class Visc, class T>
 __device__ __forceinline__ void interior_flux(const DevMesh& m, const T* __restrict__ V,
                                               const T* __restrict__ grad,
                                               const T* __restrict__ phi, const T* __restrict__ fv, int f,
-                                              int L, int sL, int Rr, int sR, const Visc& visc, T* F) {
+                                              int L, int sL, int Rr, int sR, const Inv& inv, const Visc& visc,
+                                              T* F) {
   constexpr bool kFaceVals = kSecondOrder && !Visc::isViscous;
   constexpr bool kRecon = kSecondOrder && !kFaceVals;  // extrapolated here
   constexpr bool kGrad = Visc::isViscous;
@@ -525,8 +518,7 @@ __device__ __forceinline__ void interior_flux(const DevMesh& m, const T* __restr
       vr[k] += tr * pr[k];
     }
   }
-  const ma::RoeFlux roe;
-  roe.flux_frame(vl, vr, F, fr);
+  ma::inviscid_flux(inv, vl, vr, F, fr);
   if (Visc::isViscous) {
     T vf[5], gf[5][3], fv[5];
 #pragma unroll
@@ -542,78 +534,48 @@ __device__ __forceinline__ void interior_flux(const DevMesh& m, const T* __restr
 }
 
 // Boundary faces, one thread each: the contribution each adds to its cell's
-// flux sum -- -F for Extrapolate_BC.h:69-101, Tangent_BC.h:71-122 and
-// Inflow_BC.h:72-110, -F_inviscid + F_viscous for NoSlip_BC.h:76-156 (whose
-// newtonian viscous part is applied even in inviscid runs,
-// TimeSolverExplicitRK4.h:457-460).  Written to bc[5][nbf].
-template <class T>
-__device__ __forceinline__ void bc_face(const DevMesh& m, const T* __restrict__ V, double prandtl,
-                                        const Inflow* __restrict__ inflows, const int* __restrict__ bc_case,
-                                        T* __restrict__ bc, int i) {
+// flux sum, from the boundary policy of the face's kind (BCs::Extrapolate,
+// Tangent, NoSlip, Inflow; the reference's functors Extrapolate_BC.h:69-101,
+// Tangent_BC.h:71-122, NoSlip_BC.h:76-156, Inflow_BC.h:72-110) with the
+// inviscid policy; the no-slip wall's viscous part is always the Newtonian
+// flux, even in inviscid runs (TimeSolverExplicitRK4.h:457-460).  Written to
+// bc[5][nbf].
+template <class Inv, class BCs, class T>
+__device__ __forceinline__ void bc_face(const DevMesh& m, const T* __restrict__ V, const Inv& inv,
+                                        double prandtl, const Inflow* __restrict__ inflows,
+                                        const int* __restrict__ bc_case, T* __restrict__ bc, int i) {
   const int nbf = m.nf - m.nif;
   const int f = m.nif + i;
   const int c = m.fl[f];
   const int kind = m.fkind[f];
-  T vc[5], vr[5], F[5];
+  T vc[5], out[5];
   load5(V, m.nc, c, vc);
   const ma::FaceFrameT<T> fr = load_frame<T>(m, f);
-  const ma::RoeFlux roe;
-  T out[5];
-  if (kind == MA_BC_EXTRAPOLATE) {
-    roe.flux_frame(vc, vc, F, fr);
-#pragma unroll
-    for (int k = 0; k < 5; ++k) out[k] = -F[k];
-  } else if (kind == MA_BC_INFLOW) {
-    double vi[5];
-    ma::primitives(inflows[bc_case[i]].U, vi);
-#pragma unroll
-    for (int k = 0; k < 5; ++k) vr[k] = T(vi[k]);
-    roe.flux_frame(vc, vr, F, fr);
-#pragma unroll
-    for (int k = 0; k < 5; ++k) out[k] = -F[k];
-  } else {
-    // mirror state; area_norm (Tangent_BC.h:87-91) equals |n| of the frame bit for bit
-    const T* n = fr.n;
-    const T an = fr.nn;
-    T ub = 0.0;
-    ub += vc[1] * n[0] / an;
-    ub += vc[2] * n[1] / an;
-    ub += vc[3] * n[2] / an;
-    vr[0] = vc[0];
-    vr[1] = vc[1] - T(2) * ub * n[0] / an;
-    vr[2] = vc[2] - T(2) * ub * n[1] / an;
-    vr[3] = vc[3] - T(2) * ub * n[2] / an;
-    vr[4] = vc[4];
-    roe.flux_frame(vc, vr, F, fr);
-    if (kind == MA_BC_TANGENT) {
-#pragma unroll
-      for (int k = 0; k < 5; ++k) out[k] = -F[k];
-    } else {
-      // no-slip wall: viscous flux from the one-sided gradient to the wall
-      // state (rho, 0, 0, 0, T) over the centroid-to-face distance
-      const T vf[5] = {vc[0], T(0.0), T(0.0), T(0.0), vc[4]};
-      const T inv = T(m.fwall[f]);
-      T gf[5][3], fv[5];
-#pragma unroll
-      for (int k = 0; k < 5; ++k)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) gf[k][d] = (vf[k] - vc[k]) * fr.nu[d] * inv;
-      const ma::NewtonianViscousFlux wall{prandtl};
-      wall.compute_flux(gf, vf, n, fv);
-#pragma unroll
-      for (int k = 0; k < 5; ++k) out[k] = -F[k] + fv[k];
-    }
+  const ma::NewtonianViscousFlux wallv{prandtl};
+  switch (kind) {
+    case MA_BC_EXTRAPOLATE:
+      typename BCs::Extrapolate{}(vc, fr, nullptr, T(0), inv, wallv, out);
+      break;
+    case MA_BC_TANGENT:
+      typename BCs::Tangent{}(vc, fr, nullptr, T(0), inv, wallv, out);
+      break;
+    case MA_BC_INFLOW:
+      typename BCs::Inflow{}(vc, fr, inflows[bc_case[i]].U, T(0), inv, wallv, out);
+      break;
+    default:  // MA_BC_NOSLIP (kinds are checked at setup)
+      typename BCs::NoSlip{}(vc, fr, nullptr, T(m.fwall[f]), inv, wallv, out);
+      break;
   }
 #pragma unroll
   for (int k = 0; k < 5; ++k) bc[(size_t)k * nbf + i] = out[k];
 }
 
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const T* __restrict__ V, double prandtl,
+template <class Inv, class T>
+__global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const T* __restrict__ V, Inv inv, double prandtl,
                                                const Inflow* __restrict__ inflows,
                                                const int* __restrict__ bc_case, T* __restrict__ bc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m.nf - m.nif) bc_face(m, V, prandtl, inflows, bc_case, bc, i);
+  if (i < m.nf - m.nif) bc_face<Inv, ma::ReferenceBoundaries>(m, V, inv, prandtl, inflows, bc_case, bc, i);
 }
 
 // Residual and RK accumulate of one tile of at most kTile cells:
@@ -628,7 +590,7 @@ __global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const T* __restrict__ 
 // Slot entries: (index << 3) | code, code 0 = left side, 1 = right side of an
 // internal face (index = LDS slot), 4 + kind - 1 = boundary face of that kind
 // (index = boundary-face number).
-template <bool kSecondOrder, class Visc, class T>
+template <bool kSecondOrder, class Inv, class Visc, class T>
 // Un, Vnext, alpha_next, last: with Vnext set, the cell's primitives for the
 // next stage, primitives(base + alpha_next R) -- update<Device> and
 // ComputePrimitives of that stage, the same arithmetic as k_prim -- are
@@ -640,9 +602,9 @@ __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, in
                                           const T* __restrict__ phi, const T* __restrict__ fv,
                                           const T* __restrict__ bc,
                                           const double* Uacc_in, double* Uacc_out, double* __restrict__ R,
-                                          const double* __restrict__ dt_dev, double beta, const Visc& visc,
-                                          const double* __restrict__ Un, T* __restrict__ Vnext, double alpha_next,
-                                          int last) {
+                                          const double* __restrict__ dt_dev, double beta, const Inv& inv,
+                                          const Visc& visc, const double* __restrict__ Un,
+                                          T* __restrict__ Vnext, double alpha_next, int last) {
   const int f0 = t.ptr[tile];
   const int nfl = t.ptr[tile + 1] - f0;
   const int mf = t.max_faces;
@@ -663,9 +625,8 @@ __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, in
     T F[5];
     // slots are packed into the cell entries only for the face-value scheme
     constexpr bool kFaceVals = kSecondOrder && !Visc::isViscous;
-    interior_flux<kSecondOrder, Visc>(m, V, grad, phi, fv, fc, kFaceVals ? cell_of(lrc.x) : lrc.x,
-                                      slot_of(lrc.x), kFaceVals ? cell_of(lrc.y) : lrc.y, slot_of(lrc.y),
-                                      visc, F);
+    interior_flux<kSecondOrder>(m, V, grad, phi, fv, fc, kFaceVals ? cell_of(lrc.x) : lrc.x, slot_of(lrc.x),
+                                kFaceVals ? cell_of(lrc.y) : lrc.y, slot_of(lrc.y), inv, visc, F);
 #pragma unroll
     for (int k = 0; k < 5; ++k) lds[k * mf + i] = F[k];
   }
@@ -745,7 +706,7 @@ __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, in
 template <class T>
 constexpr int kFluxWaves = sizeof(T) == 8 ? 3 : 1;
 
-template <bool kSecondOrder, class Visc, class T>
+template <bool kSecondOrder, class Inv, class Visc, class T>
 __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kFluxWaves<T>)))
 void k_flux(DevMesh m, TileMap t, const T* __restrict__ V,
             const T* __restrict__ grad,
@@ -754,7 +715,7 @@ void k_flux(DevMesh m, TileMap t, const T* __restrict__ V,
             const double* Uacc_in,  // may alias Uacc_out
             double* Uacc_out, double* __restrict__ R,
             const double* __restrict__ dt_dev, double beta,
-            Visc visc, const double* __restrict__ Un, T* Vnext,
+            Inv inv, Visc visc, const double* __restrict__ Un, T* Vnext,
             double alpha_next, int last) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* lds = reinterpret_cast<T*>(lds_raw);  // 5 planes of t.max_faces
@@ -768,8 +729,8 @@ void k_flux(DevMesh m, TileMap t, const T* __restrict__ V,
     const int tile = xcd * t.per_xcd + j;
     if (tile >= t.ntiles) break;  // uniform over the workgroup
     // the face-value scheme reads no gradients: its face values come in grad
-    flux_tile<kSecondOrder, Visc, T>(m, t, tile, lds, V, grad, phi, grad, bc, Uacc_in, Uacc_out, R, dt_dev,
-                                     beta, visc, Un, Vnext, alpha_next, last);
+    flux_tile<kSecondOrder, Inv, Visc, T>(m, t, tile, lds, V, grad, phi, grad, bc, Uacc_in, Uacc_out, R, dt_dev,
+                                          beta, inv, visc, Un, Vnext, alpha_next, last);
     __syncthreads();  // the LDS is reused by the next tile
   }
 }
@@ -973,7 +934,8 @@ __global__ __launch_bounds__(kBlock) void k_prim_multi(const CaseView* __restric
 }
 
 template <bool kSecondOrder, bool kFaceVals, class Limiter, class T>
-__global__ __launch_bounds__(kBlock) void k_grad_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGradWaves<kSecondOrder, T>)))
+void k_grad_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
                                                        int n, int per_xcd, int vsel, int store_grad) {
   const int g = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
   if (g >= start[n]) return;
@@ -987,22 +949,23 @@ __global__ __launch_bounds__(kBlock) void k_grad_multi(const CaseView* __restric
                                                    static_cast<T*>(v.fv), c);
 }
 
-template <class T>
+template <class Inv, class T>
 __global__ __launch_bounds__(kBlock) void k_bc_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
-                                                     int n, double prandtl, int vsel) {
+                                                     int n, Inv inv, double prandtl, int vsel) {
   const int g = blockIdx.x;
   const int k = find_case(start, n, g);
   const CaseView& v = cv[k];
   const int i = (g - start[k]) * kBlock + threadIdx.x;
   if (i < v.m.nf - v.m.nif)
-    bc_face(v.m, static_cast<const T*>(v.V[vsel]), prandtl, v.inflow, v.bc_case, static_cast<T*>(v.bc), i);
+    bc_face<Inv, ma::ReferenceBoundaries>(v.m, static_cast<const T*>(v.V[vsel]), inv, prandtl, v.inflow,
+                                          v.bc_case, static_cast<T*>(v.bc), i);
 }
 
-template <bool kSecondOrder, class Visc, class T>
+template <bool kSecondOrder, class Inv, class Visc, class T>
 __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kFluxWaves<T>)))
 void k_flux_multi(const CaseView* __restrict__ cv, const int* __restrict__ start,
                   int n, int per_xcd, int parity, int irk, double beta,
-                  double alpha_next, Visc visc) {
+                  double alpha_next, Inv inv, Visc visc) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int g = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
   if (g >= start[n]) return;  // whole workgroup, before any barrier
@@ -1010,11 +973,11 @@ void k_flux_multi(const CaseView* __restrict__ cv, const int* __restrict__ start
   const CaseView& v = cv[k];
   const int cur = v.parity ^ parity;
   double* Uacc = v.U[1 - cur];
-  flux_tile<kSecondOrder, Visc, T>(v.m, v.t, g - start[k], reinterpret_cast<T*>(lds_raw),
+  flux_tile<kSecondOrder, Inv, Visc, T>(v.m, v.t, g - start[k], reinterpret_cast<T*>(lds_raw),
                                    static_cast<const T*>(v.V[irk & 1]), static_cast<const T*>(v.grad),
                                    static_cast<const T*>(v.phi), static_cast<const T*>(v.fv),
                                    static_cast<const T*>(v.bc),
-                                   irk == 0 ? v.U[cur] : Uacc, Uacc, v.R, v.dt, beta, visc, v.U[cur],
+                                   irk == 0 ? v.U[cur] : Uacc, Uacc, v.R, v.dt, beta, inv, visc, v.U[cur],
                                    static_cast<T*>(v.V[1 - (irk & 1)]), alpha_next, irk == 3);
 }
 
@@ -1432,7 +1395,7 @@ void merge_layouts(std::vector<Layout>& parts, Layout& M, std::vector<int>& tile
   M.fkind.assign(nf, 0);
   M.fwall.assign(nf, 0.0);
   M.fn.assign((size_t)3 * nf, 0.0);
-  if (!M.derive_frame) M.ffr.assign((size_t)10 * nf, 0.0);
+  if (!M.derive_frame) M.ffr.assign((size_t)6 * nf, 0.0);
   M.fx.assign((size_t)3 * nf, 0.0);
   M.vol.assign(nc, 0.0);
   M.xc.assign((size_t)3 * nc, 0.0);
@@ -1478,12 +1441,14 @@ void merge_layouts(std::vector<Layout>& parts, Layout& M, std::vector<int>& tile
       if (!M.derive_frame) {
         if (L.derive_frame) {  // this part stored no frames: they follow from the area vector
           const double a[3] = {L.fn[f], L.fn[(size_t)lnf + f], L.fn[(size_t)2 * lnf + f]};
-          const ma::FaceFrame fr = ma::frame_from_area(a);
-          const double v[10] = {fr.nn, fr.nu[0], fr.nu[1], fr.nu[2], fr.tu[0],
-                                fr.tu[1], fr.tu[2], fr.bu[0], fr.bu[1], fr.bu[2]};
-          for (int d = 0; d < 10; ++d) M.ffr[(size_t)d * nf + g] = v[d];
+          double t[3], b[3];
+          ma::tangent_binormal(a, t, b);
+          for (int d = 0; d < 3; ++d) {
+            M.ffr[(size_t)d * nf + g] = t[d];
+            M.ffr[(size_t)(3 + d) * nf + g] = b[d];
+          }
         } else {
-          for (int d = 0; d < 10; ++d) M.ffr[(size_t)d * nf + g] = L.ffr[(size_t)d * lnf + f];
+          for (int d = 0; d < 6; ++d) M.ffr[(size_t)d * nf + g] = L.ffr[(size_t)d * lnf + f];
         }
       }
       if (f >= lnif) bc_case[bo[k] + (f - lnif)] = k;
@@ -1578,14 +1543,10 @@ ma_status plan_layout(const ma_mesh_data* md, const TileShape& shape, Layout& L)
       fx[(size_t)d * nf + f] = F.coordinates[3 * i + d];
     }
   };
-  auto put_frame = [&](int f, const ma_faces& F, int i) {
-    const ma::FaceFrame fr = ma::make_frame(&F.face_normal[3 * i], &F.face_tangent[3 * i],
-                                            &F.face_binormal[3 * i]);
-    ffr[f] = fr.nn;
+  auto put_frame = [&](int f, const ma_faces& F, int i) {  // the caller's tangent and binormal
     for (int d = 0; d < 3; ++d) {
-      ffr[(size_t)(1 + d) * nf + f] = fr.nu[d];
-      ffr[(size_t)(4 + d) * nf + f] = fr.tu[d];
-      ffr[(size_t)(7 + d) * nf + f] = fr.bu[d];
+      ffr[(size_t)d * nf + f] = F.face_tangent[3 * i + d];
+      ffr[(size_t)(3 + d) * nf + f] = F.face_binormal[3 * i + d];
     }
   };
   for (int g = 0; g < nif; ++g) {
@@ -1612,8 +1573,8 @@ ma_status plan_layout(const ma_mesh_data* md, const TileShape& shape, Layout& L)
       }
     }
   }
-  if (!L.derive_frame) {  // stored frames (|n|, unit n, t, b) from the caller's vectors
-    ffr.assign((size_t)10 * nf, 0.0);
+  if (!L.derive_frame) {  // stored tangents and binormals, the caller's vectors
+    ffr.assign((size_t)6 * nf, 0.0);
     for (int g = 0; g < nif; ++g) put_frame(g, I, dface[g]);
     int f = nif;
     for (int g = 0; g < md->num_boundary_groups; ++g) {
@@ -2008,7 +1969,7 @@ inline int resident(const ma_solver* s, int per) {
   return std::max(1, std::min(per, s->wg_per_cu * std::max(1, s->cus / kXcds)));
 }
 
-template <bool SO, class Visc, class Lim, class T>
+template <bool SO, class Inv, class Visc, class Lim, class T>
 void run_stage(ma_solver* s, int irk) {
   // TimeSolverExplicitRK4.h:187-195
   constexpr double alpha[4] = {0.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};
@@ -2042,27 +2003,29 @@ void run_stage(ma_solver* s, int irk) {
     });
   }
   Visc visc{effective_prandtl(s->opt)};
+  const Inv inv{};
   // Stage 0 accumulates from sol_n (sol_np1 == sol_n at the start of a step).
   const double* acc_in = irk == 0 ? Un : Uacc;
   const int nbf = m.nf - m.nif;
   if (nbf > 0) {
     s->launch("k_bc", [&] {
-      hipLaunchKernelGGL(k_bc<T>, dim3(nblocks(nbf)), dim3(kBlock), 0, s->stream, m,
-                         static_cast<const T*>(Vs), visc.prandtl, s->inflow_dev, s->bc_case,
+      hipLaunchKernelGGL((k_bc<Inv, T>), dim3(nblocks(nbf)), dim3(kBlock), 0, s->stream, m,
+                         static_cast<const T*>(Vs), inv, visc.prandtl, s->inflow_dev, s->bc_case,
                          static_cast<T*>(s->bc));
     });
   }
   const size_t lds = (size_t)s->t.max_faces * 5 * sizeof(T);
   s->launch("k_flux", [&] {
-    hipLaunchKernelGGL((k_flux<SO, Visc, T>), dim3(resident(s, s->t.per_xcd) * kXcds), dim3(kTile), lds, s->stream, m,
-                       s->t, static_cast<const T*>(Vs), static_cast<const T*>(FV ? s->fv : s->grad),
+    hipLaunchKernelGGL((k_flux<SO, Inv, Visc, T>), dim3(resident(s, s->t.per_xcd) * kXcds), dim3(kTile), lds,
+                       s->stream, m, s->t, static_cast<const T*>(Vs), static_cast<const T*>(FV ? s->fv : s->grad),
                        static_cast<const T*>(s->phi), static_cast<const T*>(s->bc), acc_in, Uacc, s->R,
-                       s->dt_dev, beta[irk], visc, Un, Vn, irk < 3 ? alpha[irk + 1] : alpha[0], irk == 3 ? 1 : 0);
+                       s->dt_dev, beta[irk], inv, visc, Un, Vn, irk < 3 ? alpha[irk + 1] : alpha[0],
+                       irk == 3 ? 1 : 0);
   });
   if (irk == 3) s->v0_valid = true;  // V now holds the next step's stage-0 primitives
 }
 
-template <bool SO, class Visc, class Lim, class T>
+template <bool SO, class Inv, class Visc, class Lim, class T>
 void run_steps(ma_solver* s, int nsteps) {
   for (int it = 0; it < nsteps; ++it) {
     // per-case dt (CFL minimum or fixed) and time += dt
@@ -2076,18 +2039,24 @@ void run_steps(ma_solver* s, int nsteps) {
       hipLaunchKernelGGL(k_dt_final, dim3(s->ncases), dim3(kBlock), 0, s->stream, s->partial, s->case_tiles,
                          s->cfl_dev, s->dt_dev, s->time_dev);
     });
-    for (int irk = 0; irk < 4; ++irk) run_stage<SO, Visc, Lim, T>(s, irk);
+    for (int irk = 0; irk < 4; ++irk) run_stage<SO, Inv, Visc, Lim, T>(s, irk);
     s->cur = 1 - s->cur;  // U_acc after the last stage is the new sol_n
     ++s->steps_done;
   }
 }
 
+// The policies of a run: inviscid flux (the reference's Roe_Flux.h; the only
+// one this library instantiates), viscous flux and limiter selected from the
+// options, as TimeSolverExplicitRK4 instantiates compute_face_flux and the
+// limiter (TimeSolverExplicitRK4.h:388-486, StencilLimiter.h:455).
+using Inviscid = ma::RoeFlux;
+
 template <bool SO, class Visc, class T>
 void dispatch_lim(ma_solver* s, int n) {
   if (s->opt.limiter == MA_LIMITER_VANALBADA)
-    run_steps<SO, Visc, ma::VanAlbadaLimiter, T>(s, n);
+    run_steps<SO, Inviscid, Visc, ma::VanAlbadaLimiter, T>(s, n);
   else
-    run_steps<SO, Visc, ma::VenkatLimiter, T>(s, n);
+    run_steps<SO, Inviscid, Visc, ma::VenkatLimiter, T>(s, n);
 }
 
 template <class T>
@@ -2898,7 +2867,7 @@ ma_status multi_set(Multi* M, ma_solver* const* cases, const int* parity, int n)
 
 namespace {
 
-template <bool SO, class Visc, class Lim, class T>
+template <bool SO, class Inv, class Visc, class Lim, class T>
 void multi_step_t(ma::Multi* M, int parity) {
   constexpr double alpha[4] = {0.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};
   constexpr double beta[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
@@ -2912,6 +2881,7 @@ void multi_step_t(ma::Multi* M, int parity) {
                        parity, M->scheme.viscous, prandtl);
   hipLaunchKernelGGL(k_dt_final_multi, dim3(n), dim3(kBlock), 0, M->stream, M->d_views);
   Visc visc{prandtl};
+  const Inv inv{};
   const int per_cells = (M->totals[0] + kXcds - 1) / kXcds;
   const int per_tiles = (M->totals[2] + kXcds - 1) / kXcds;
   const size_t lds = (size_t)M->max_faces * 5 * sizeof(T);
@@ -2924,20 +2894,20 @@ void multi_step_t(ma::Multi* M, int parity) {
                          dim3(kBlock), 0, M->stream, M->d_views, sc, n, per_cells, irk & 1,
                          Visc::isViscous ? 1 : 0);
     if (M->totals[1] > 0)
-      hipLaunchKernelGGL(k_bc_multi<T>, dim3(M->totals[1]), dim3(kBlock), 0, M->stream, M->d_views, sb, n,
-                         prandtl, irk & 1);
-    hipLaunchKernelGGL((k_flux_multi<SO, Visc, T>), dim3(per_tiles * kXcds), dim3(kTile), lds, M->stream,
+      hipLaunchKernelGGL((k_bc_multi<Inv, T>), dim3(M->totals[1]), dim3(kBlock), 0, M->stream, M->d_views, sb, n,
+                         inv, prandtl, irk & 1);
+    hipLaunchKernelGGL((k_flux_multi<SO, Inv, Visc, T>), dim3(per_tiles * kXcds), dim3(kTile), lds, M->stream,
                        M->d_views, stl, n, per_tiles, parity, irk, beta[irk], irk < 3 ? alpha[irk + 1] : 0.0,
-                       visc);
+                       inv, visc);
   }
 }
 
 template <bool SO, class Visc, class T>
 void multi_lim(ma::Multi* M, int parity) {
   if (M->scheme.limiter == MA_LIMITER_VANALBADA)
-    multi_step_t<SO, Visc, ma::VanAlbadaLimiter, T>(M, parity);
+    multi_step_t<SO, Inviscid, Visc, ma::VanAlbadaLimiter, T>(M, parity);
   else
-    multi_step_t<SO, Visc, ma::VenkatLimiter, T>(M, parity);
+    multi_step_t<SO, Inviscid, Visc, ma::VenkatLimiter, T>(M, parity);
 }
 
 template <class T>
