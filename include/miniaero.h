@@ -190,6 +190,18 @@ ma_status ma_solver_get_time(ma_solver* s, double* time, double* last_dt);
  * the device in a fixed order (repeatable bit for bit).  No reference
  * counterpart (a convergence monitor). */
 ma_status ma_solver_residual_norms(ma_solver* s, double* norms);
+/* The same norms, bit for bit, for callers that cannot allocate device
+ * memory or block: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  No counterpart in the reference.
+ * `norms` is a DEVICE array of 5 * ncases doubles; the call queues two
+ * kernels on the solver's stream and returns without waiting (the values are
+ * there once the stream reaches them).  It allocates nothing: the per-tile
+ * partial sums go to a static device array of at most 262,144 tiles (10 MiB
+ * per GPU, held until the process ends); a solver with more flux tiles gets
+ * MA_ERR_INVALID.  That array is shared by every call on the GPU: calls that
+ * overlap on one GPU (other streams or threads) race, calls queued in order
+ * on one stream do not. */
+ma_status ma_solver_residual_norms_noalloc(ma_solver* s, double* norms);
 /* Sets *nonfinite to the number of non-finite values in the state. */
 ma_status ma_solver_check_finite(ma_solver* s, long long* nonfinite);
 /* The HIP stream (hipStream_t) all of the solver's work is queued on. */

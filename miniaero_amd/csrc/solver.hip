@@ -63,6 +63,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -2545,6 +2546,61 @@ extern "C" ma_status ma_solver_residual_norms(ma_solver* s, double* norms) {
   MA_HIP(hipMemcpyAsync(norms, s->norm_out, (size_t)5 * s->ncases * sizeof(double), hipMemcpyDeviceToHost,
                         s->stream));
   MA_HIP(hipStreamSynchronize(s->stream));
+  return MA_OK;
+}
+
+// ------------------------------------------- residual norms, no allocation --
+// ma_solver_residual_norms_noalloc keeps its per-tile partial sums in this
+// file-scope array instead of the solver's own buffer.  Every GPU the library
+// runs on holds its own copy until the process ends.  In host code the name
+// `scratch` is only a stand-in for the symbol, not a device address: the
+// address on each device is looked up once per device with
+// hipGetSymbolAddress and kept in g_scratch, and only that pointer reaches
+// kernels.  All calls on one GPU share its copy, so calls that overlap on one
+// GPU (different streams, or threads) race; calls queued on one stream do not.
+// Nothing in it needs to start at zero: k_norm_partial writes all 5 sums of
+// every tile before k_norm_final reads them, on the same stream.
+constexpr int kScratchTiles = 1 << 18;  // 5 doubles per tile: 10 MiB per GPU
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) double scratch[(size_t)5 * kScratchTiles];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch[kScratchDevices] = {};
+
+// Device address of `scratch` on `device`: looked up on the first call for
+// that device, with that device current (and the caller's device restored).
+static ma_status scratch_on(int device, double** out) {
+  if (device < 0 || device >= kScratchDevices)
+    return ma::fail(MA_ERR_UNSUPPORTED, "residual norms without allocation: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    int prev = -1;
+    MA_HIP(hipGetDevice(&prev));
+    if (prev != device) MA_HIP(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (prev != device) (void)hipSetDevice(prev);
+    if (e != hipSuccess || !p)
+      return ma::fail(MA_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch[device];
+  return MA_OK;
+}
+
+extern "C" ma_status ma_solver_residual_norms_noalloc(ma_solver* s, double* norms) {
+  if (!s || !norms) return ma::fail(MA_ERR_INVALID, "null argument");
+  if (s->t.ntiles > kScratchTiles)
+    return ma::fail(MA_ERR_INVALID, "ma_solver_residual_norms_noalloc: " + std::to_string(s->t.ntiles) +
+                                        " flux tiles, more than the " + std::to_string(kScratchTiles) +
+                                        " its static scratch holds; use ma_solver_residual_norms");
+  double* part = nullptr;
+  if (ma_status st = scratch_on(s->device, &part)) return st;
+  if (s->t.ntiles > 0)
+    hipLaunchKernelGGL(k_norm_partial, dim3(s->t.ntiles), dim3(kTile), 0, s->stream, s->t, s->m.nc, s->R, part);
+  hipLaunchKernelGGL(k_norm_final, dim3((5 * s->ncases + 255) / 256), dim3(256), 0, s->stream, part,
+                     s->case_tiles, s->ncases, norms);
+  MA_HIP(hipGetLastError());
   return MA_OK;
 }
 
