@@ -1,0 +1,190 @@
+"""The batched (512 cases) and served (arrival stream) workloads at their own
+sizes, against the oracle.
+
+C4, the whole 512-case batch as the benchmark runs it (autotuned tile shape,
+100 RK4 steps, HIP graphs of 2 steps and of 100 steps):
+  * FP64: both graph runs give the same bits; every one of the 512 cases
+    equals, bit for bit, its result from four other batches (the cases dealt
+    round-robin, plain 256-cell tiles, no graphs), so no case depends on its
+    neighbours, the tile shape or the graphs; the four smallest cases (8.8k
+    cells) equal the oracle's 100-step run bit for bit.
+
+C5, the first 40 requests of the seeded served stream (4k-1.9M cells, 59-496
+steps, viscous and inviscid, FP32 arithmetic), replayed at their arrival
+times so that they join while others run:
+  * every served result equals its run alone (TimeSolverExplicitRK4), bit
+    for bit;
+  * FP64 variants of the three smallest requests, served in a mix with
+    larger FP32 cases, equal the oracle bit for bit; their FP32 results are
+    within 1e-5 of the oracle, per conserved variable.
+"""
+import concurrent.futures as cf
+import dataclasses
+import time
+
+import numpy as np
+import pytest
+
+import miniaero_amd as M
+from miniaero_amd import _lib
+from miniaero_amd.configs import batched_c4, served_c5
+from miniaero_amd.mesh import generate_meshes
+from oracle import oracle as O
+from test_gpu_parity import _log, precision_errors
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP32 = 1e-5
+
+
+def _kw(opts):
+    return {f.name: getattr(opts, f.name) for f in dataclasses.fields(opts)
+            if f.name not in ("output_results", "output_frequency")}
+
+
+def _oracle_runs(opts_list):
+    """Oracle states after each case's ntimesteps; meshes are built one at a
+    time (the generator's shuffle uses the C library's rand() state), the
+    steps on threads (the library releases the GIL)."""
+    solvers = []
+    for o in opts_list:
+        oo = O.make_options(**_kw(o))
+        s = O.OracleSolver(O.OracleMesh(oo))
+        s.init()
+        solvers.append((s, o.ntimesteps))
+
+    def run(sn):
+        s, n = sn
+        s.step(n)
+        return s.state()
+
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(solvers))) as ex:
+        return list(ex.map(run, solvers))
+
+
+def _batch_states(meshes, cases, steps, graph=0):
+    b = M.BatchSolver(list(zip(meshes, cases)))
+    try:
+        b.initialize()
+        if graph:
+            b.set_graph(graph)
+        b.step(steps)
+        return [b.case_state(k) for k in range(len(cases))], b.tuning()
+    finally:
+        b.close()
+
+
+@pytest.mark.slow
+def test_c4_whole_batch_fp64_bitwise():
+    cases = batched_c4()
+    assert len(cases) == 512 and all(o.autotune == 1 and o.ntimesteps == 100 for o in cases)
+    t0 = time.time()
+    meshes = generate_meshes(cases)
+    _log(f"[c4] 512 meshes, {sum(o.nx * o.ny * o.nz for o in cases)} cells, {time.time() - t0:.1f} s")
+
+    g2, tune = _batch_states(meshes, cases, 100, graph=2)
+    _log(f"[c4] FP64, graphs of 2 steps, tuning {tune}")
+    g100, _ = _batch_states(meshes, cases, 100, graph=100)
+    _log("[c4] FP64, one graph of 100 steps")
+    for k in range(512):
+        assert np.array_equal(g2[k], g100[k]), f"case {k}: graphs of 2 and of 100 steps differ"
+    del g100
+
+    # the same cases in four other batches: dealt round-robin, consecutive-cell tiles, no graphs
+    for r in range(4):
+        idx = list(range(r, 512, 4))
+        other, _ = _batch_states([meshes[k] for k in idx],
+                                 [cases[k].replace(autotune=0, tile_shape=(-1, 0, 0)) for k in idx], 100)
+        for j, k in enumerate(idx):
+            assert np.array_equal(other[j], g2[k]), f"case {k} differs between batches"
+        _log(f"[c4] sub-batch {r}: 128 cases equal the full batch bit for bit")
+
+    small = sorted(range(512), key=lambda k: cases[k].nx * cases[k].ny * cases[k].nz)[:4]
+    ref = _oracle_runs([cases[k] for k in small])
+    for k, U in zip(small, ref):
+        o = cases[k]
+        same = np.array_equal(g2[k], U)
+        _log(f"[c4] case {k} ({o.nx}x{o.ny}x{o.nz}) vs oracle, 100 steps: bitwise={same}")
+        assert same, f"case {k} differs from the oracle"
+
+
+def _serve(stream, max_active=32, budget=200e9):
+    """Submits each request at its arrival time; returns {index: (result, U)}."""
+    srv = M.Server(hbm_budget_bytes=budget, max_active=max_active, setup_threads=8)
+    out = {}
+    try:
+        t0 = srv.now()
+        ids = {}
+        for i, (arrival, o) in enumerate(stream):
+            while srv.now() < t0 + arrival:
+                time.sleep(0.001)
+            ids[srv.submit(o)] = i
+        t_end = time.time() + 300
+        while len(out) < len(ids) and time.time() < t_end:
+            for r in srv.poll(timeout=1.0):
+                U = srv.state(r["id"], r["cells"]) if r["status"] == _lib.MA_OK else None
+                out[ids[r["id"]]] = (r, U)
+                srv.release(r["id"])
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert len(out) == len(stream), f"only {len(out)} of {len(stream)} requests finished"
+    assert st["active"] == 0 and st["waiting"] == 0 and st["region_used"] == 0
+    return out
+
+
+def _alone(o):
+    md = M.Parallel3DMesh(o.nx, o.ny, o.nz, o.lx, o.ly, o.lz, o.problem_type, o.angle,
+                          options=o).fillMeshData()
+    s = M.TimeSolverExplicitRK4(md, o)
+    try:
+        s.initialize()
+        s.step(o.ntimesteps)
+        return s.state()
+    finally:
+        s.close()
+
+
+@pytest.mark.slow
+def test_c5_stream_prefix_served_equals_alone():
+    stream = served_c5()[:40]
+    cells = [o.nx * o.ny * o.nz for _, o in stream]
+    assert max(cells) > 1_800_000 and min(o.ntimesteps for _, o in stream) < 100
+    assert max(o.ntimesteps for _, o in stream) > 450
+    assert {o.viscous for _, o in stream} == {0, 1}
+    t0 = time.time()
+    out = _serve(stream)
+    _log(f"[c5] 40 requests served in {time.time() - t0:.1f} s")
+    lat = sorted(r["t_finish"] - r["t_submit"] for r, _ in out.values())
+    _log(f"[c5] latency p50 {lat[len(lat) // 2]:.2f} s, max {lat[-1]:.2f} s")
+    for i, (arrival, o) in enumerate(stream):
+        r, U = out[i]
+        assert r["status"] == _lib.MA_OK and r["steps"] == o.ntimesteps, r
+        same = np.array_equal(U, _alone(o))
+        _log(f"[c5] request {i}: {cells[i]} cells, {o.ntimesteps} steps, viscous {o.viscous}: "
+             f"served == alone {same}")
+        assert same, f"request {i} differs from its run alone"
+
+
+@pytest.mark.slow
+def test_c5_small_requests_against_oracle():
+    stream = served_c5()[:40]
+    small = sorted(range(40), key=lambda i: stream[i][1].nx * stream[i][1].ny * stream[i][1].nz)[:3]
+    assert {stream[i][1].viscous for i in small} == {0, 1}
+    # the three small ones in FP64 and in FP32, mixed with four larger FP32 requests
+    mix = []
+    for i in small:
+        mix.append((0.0, stream[i][1].replace(precision=0)))
+        mix.append((0.0, stream[i][1]))
+    mix += [(0.002 * j, stream[i][1]) for j, i in enumerate((0, 3, 5, 10))]
+    out = _serve(mix, max_active=6)
+    ref = _oracle_runs([stream[i][1] for i in small])
+    for j, i in enumerate(small):
+        o = stream[i][1]
+        (_, U64), (_, U32) = out[2 * j], out[2 * j + 1]
+        same = np.array_equal(U64, ref[j])
+        whole, per, _ = precision_errors(U32, ref[j])
+        _log(f"[c5] request {i} ({o.nx}x{o.ny}x{o.nz}, {o.ntimesteps} steps, viscous {o.viscous}): "
+             f"FP64 served == oracle {same}; FP32 vs oracle {whole:.2e}, per variable max {max(per):.2e}")
+        assert same, f"request {i}: FP64 served result differs from the oracle"
+        assert whole <= TOL_FP32 and max(per) <= TOL_FP32
