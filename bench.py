@@ -349,17 +349,21 @@ def bench_batch(args, M, cases, copy_gbs) -> dict:
     extra = {}
     fp32 = cases[0].precision != 0
     if args.compare_fp64 and fp32:
+        # compared after at least the cases' own step count
+        n_cmp = max(warm_total + args.steps, cases[0].ntimesteps)
+        if n_cmp > warm_total + args.steps:
+            solver.step(n_cmp - warm_total - args.steps)
         U = [solver.case_state(k) for k in range(len(cases))]
         ref = M.BatchSolver(list(zip(meshes, [o.replace(precision=0, autotune=0) for o in cases])))
         ref.initialize()
-        ref.step(warm_total + args.steps)
+        ref.step(n_cmp)
         errs = [precision_errors(U[k], ref.case_state(k)) for k in range(len(cases))]
         ref.close()
         worst = {k: max(e[k] for e in errs) for k in errs[0]}
         extra["precision_error"] = {
             "metric": "relative L2 vs the FP64 run after warmup+steps, worst case of the batch; "
                       "momentum components scaled by the momentum-vector norm",
-            "steps": warm_total + args.steps, **worst}
+            "steps": n_cmp, **worst}
         print(f"[bench] precision error: {json.dumps(extra['precision_error'])}", file=sys.stderr, flush=True)
     del meshes
     roof = roofline(solver, args.profile_steps, nc, nif, nbf, True, fp32,
@@ -445,6 +449,7 @@ def bench_served(args, M, stream, copy_gbs) -> dict:
         if time.time() - last_note > 10:
             print(f"[bench] {len(done)} of {len(opts_of)} finished", file=sys.stderr, flush=True)
             last_note = time.time()
+    plan_cache = srv.plan_cache()
     srv.close()
     bad = [r for r in done.values() if r["status"] != 0]
     if bad:
@@ -491,6 +496,7 @@ def bench_served(args, M, stream, copy_gbs) -> dict:
               "physics": "Roe, Green-Gauss, Venkat, 2nd order; half viscous flat plate, half slip wall",
               "precision": "fp32 flux/gradient/limiter, fp64 state",
               "max_active": args.max_active or 16, "setup_threads": args.setup_threads,
+              "layout_plans": {"reused": plan_cache["hits"], "planned": plan_cache["misses"]},
               "region_bytes": st0["region_size"],
               "peak_region_bytes_used": int(peak_used),
               "case_latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
@@ -612,8 +618,10 @@ def bench_suite(args, M, copy_gbs) -> dict:
         return x
 
     def batched_c4():
-        a = _sub(args, workload="batched_c4", graph=2, compare_fp64=False)
-        return _compact(bench_batch(a, M, BATCH_WORKLOADS["batched_c4"](), copy_gbs), a.steps)
+        # the mixed-precision form, with every case's error against its FP64 run after its 100 steps
+        a = _sub(args, workload="batched_c4", graph=2, compare_fp64=True)
+        cases = [o.replace(precision=1) for o in BATCH_WORKLOADS["batched_c4"]()]
+        return _compact(bench_batch(a, M, cases, copy_gbs), a.steps)
 
     def served_200():
         a = _sub(args, workload="served_c5", cases=200, compare_fp64=False)

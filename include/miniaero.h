@@ -252,6 +252,9 @@ typedef struct {
   int max_active;             /* cases on the device at once; 0 = 16 */
   int setup_threads;          /* host threads planning requests; 0 = 4 */
   int steps_in_flight;        /* RK4 steps queued ahead per scheme group; 0 = 2 */
+  long long plan_cache_bytes; /* host bytes of device layouts kept for reuse by requests of the same
+                               * mesh (box, cells, stretch, boundary kinds, tile shape); 0 = 16 GiB,
+                               * < 0 = none */
 } ma_server_config;
 
 typedef struct {
@@ -280,6 +283,9 @@ ma_status ma_server_release(ma_server* srv, long long id);
 ma_status ma_server_stats(ma_server* srv, long long* stats);
 /* Seconds since the server was created. */
 double ma_server_now(ma_server* srv);
+/* Requests whose layout came from the plan cache (*hits) and those that
+ * generated and planned their mesh (*misses). */
+ma_status ma_server_plan_cache(ma_server* srv, long long* hits, long long* misses);
 /* Cancels cases not yet admitted, waits for the active ones to finish their
  * launched steps, frees everything. */
 void ma_server_destroy(ma_server* srv);

@@ -35,12 +35,14 @@ EXPORTS = (
     "ma_batch_get_case_time", "ma_batch_plan", "ma_solver_set_graph",
     "ma_solver_tuning", "ma_solver_residual_norms", "ma_solver_residual_norms_noalloc", "ma_server_create", "ma_server_submit", "ma_server_poll",
     "ma_server_get_state", "ma_server_release", "ma_server_stats", "ma_server_now", "ma_server_destroy",
+    "ma_server_plan_cache",
 )
 
 
 class ma_server_config(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_longlong), ("max_active", ctypes.c_int),
-                ("setup_threads", ctypes.c_int), ("steps_in_flight", ctypes.c_int)]
+                ("setup_threads", ctypes.c_int), ("steps_in_flight", ctypes.c_int),
+                ("plan_cache_bytes", ctypes.c_longlong)]
 
 
 class ma_case_result(ctypes.Structure):
@@ -170,6 +172,7 @@ def load() -> ctypes.CDLL:
     L.ma_server_stats.argtypes = [P, ctypes.POINTER(ctypes.c_longlong)]
     L.ma_server_now.restype = ctypes.c_double
     L.ma_server_now.argtypes = [P]
+    L.ma_server_plan_cache.argtypes = [P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
     L.ma_server_destroy.argtypes = [P]
     L.ma_server_destroy.restype = None
     L.ma_batch_sizes.argtypes = [P, _I, _I]

@@ -30,6 +30,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <list>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -136,10 +137,88 @@ struct Group {
   std::vector<hipEvent_t> spare;
 };
 
+// Layouts of the meshes the server has planned, by what the generated mesh
+// and its layout depend on: the box (problem type, lengths, ramp angle,
+// cells per direction, y-stretch), the boundary kinds per side and the tile
+// shape.  Requests of a shape seen before skip mesh generation and layout
+// planning (about 2 us per cell on one core); least recently used layouts are
+// dropped beyond the cache's host-byte budget.
+struct PlanKey {
+  int problem_type, nx, ny, nz, bc_kind[6], tile_shape[3];
+  double lx, ly, lz, angle, y_stretch;
+  bool operator<(const PlanKey& o) const { return std::memcmp(this, &o, sizeof(PlanKey)) < 0; }
+};
+
+PlanKey plan_key(const ma_options& o) {
+  PlanKey k;
+  std::memset(&k, 0, sizeof(k));  // padding compares equal
+  k.problem_type = o.problem_type;
+  k.nx = o.nx;
+  k.ny = o.ny;
+  k.nz = o.nz;
+  for (int i = 0; i < 6; ++i) k.bc_kind[i] = o.bc_kind[i];
+  for (int i = 0; i < 3; ++i) k.tile_shape[i] = o.tile_shape[i];
+  k.lx = o.lx;
+  k.ly = o.ly;
+  k.lz = o.lz;
+  k.angle = o.angle;
+  k.y_stretch = o.y_stretch;
+  return k;
+}
+
+class PlanCache {
+ public:
+  struct Entry {
+    std::shared_ptr<const ma::LayoutPlan> lp;
+    long long cells = 0;
+    size_t bytes = 0;
+  };
+  void set_budget(size_t b) { budget_ = b; }
+  bool get(const PlanKey& k, Entry* e) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = map_.find(k);
+    if (it == map_.end()) {
+      ++misses_;
+      return false;
+    }
+    lru_.splice(lru_.begin(), lru_, it->second.second);  // most recent first
+    *e = it->second.first;
+    ++hits_;
+    return true;
+  }
+  void put(const PlanKey& k, const Entry& e) {
+    if (e.bytes > budget_) return;
+    std::lock_guard<std::mutex> lk(mu_);
+    if (map_.count(k)) return;  // planned meanwhile by another thread
+    lru_.push_front(k);
+    map_[k] = {e, lru_.begin()};
+    used_ += e.bytes;
+    while (used_ > budget_ && !lru_.empty()) {
+      auto it = map_.find(lru_.back());
+      used_ -= it->second.first.bytes;
+      map_.erase(it);
+      lru_.pop_back();
+    }
+  }
+  void counts(long long* hits, long long* misses) {
+    std::lock_guard<std::mutex> lk(mu_);
+    *hits = hits_;
+    *misses = misses_;
+  }
+
+ private:
+  std::mutex mu_;
+  std::list<PlanKey> lru_;
+  std::map<PlanKey, std::pair<Entry, std::list<PlanKey>::iterator>> map_;
+  size_t budget_ = 0, used_ = 0;
+  long long hits_ = 0, misses_ = 0;
+};
+
 }  // namespace
 
 struct ma_server {
   ma_server_config cfg{};
+  PlanCache plans;
   int device = 0;
   char* base = nullptr;
   Region region;
@@ -209,22 +288,30 @@ void setup_worker(ma_server* S) {
       r = S->queue.front();
       S->queue.pop_front();
     }
-    // plan on this thread, outside the lock
-    ma_mesh* mesh = nullptr;
+    // plan on this thread, outside the lock: the mesh's layout from the
+    // cache, or generated and planned here, then the case's own plan on it
     ma::PlannedCase* pc = nullptr;
     size_t bytes = 0;
     std::string err;
-    ma_status st = ma_mesh_generate(&r->opt, &mesh);
-    if (!st) {
-      ma_mesh_data md;
-      st = ma_mesh_view(mesh, &md);
+    ma_status st = MA_OK;
+    const PlanKey key = plan_key(r->opt);
+    PlanCache::Entry ent;
+    if (!S->plans.get(key, &ent)) {
+      ma_mesh* mesh = nullptr;
+      st = ma_mesh_generate(&r->opt, &mesh);
       if (!st) {
-        r->cells = md.num_owned_cells;
-        st = ma::plan_case(&md, &r->opt, &pc, &bytes);
+        ma_mesh_data md;
+        st = ma_mesh_view(mesh, &md);
+        if (!st) st = ma::plan_layout_shared(&md, &r->opt, &ent.lp, &ent.cells, &ent.bytes);
       }
+      if (mesh) ma_mesh_destroy(mesh);  // the layout holds its own copies
+      if (!st) S->plans.put(key, ent);
+    }
+    if (!st) {
+      r->cells = ent.cells;
+      st = ma::plan_case_with(ent.lp, &r->opt, &pc, &bytes);
     }
     if (st) err = ma::last_error();
-    if (mesh) ma_mesh_destroy(mesh);  // the plan holds its own copies
 
     {
       std::unique_lock<std::mutex> lk(S->mu);
@@ -478,6 +565,7 @@ extern "C" ma_status ma_server_create(const ma_server_config* cfg, ma_server** o
   if (c.max_active == 0) c.max_active = 16;
   if (c.setup_threads == 0) c.setup_threads = 4;
   if (c.steps_in_flight == 0) c.steps_in_flight = 2;
+  if (c.plan_cache_bytes == 0) c.plan_cache_bytes = 16ll << 30;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return ma::fail(MA_ERR_DEVICE, "no HIP device available");
@@ -491,6 +579,7 @@ extern "C" ma_status ma_server_create(const ma_server_config* cfg, ma_server** o
   ma_server* S = new (std::nothrow) ma_server();
   if (!S) return ma::fail(MA_ERR_NOMEM, "ma_server_create: out of host memory");
   S->cfg = c;
+  S->plans.set_budget(c.plan_cache_bytes > 0 ? (size_t)c.plan_cache_bytes : 0);
   (void)hipGetDevice(&S->device);
   if (hipMalloc((void**)&S->base, size) != hipSuccess) {
     delete S;
@@ -590,6 +679,12 @@ extern "C" ma_status ma_server_stats(ma_server* S, long long* stats) {
   stats[3] = S->finished;
   stats[4] = (long long)S->region.used();
   stats[5] = (long long)S->region.size();
+  return MA_OK;
+}
+
+extern "C" ma_status ma_server_plan_cache(ma_server* S, long long* hits, long long* misses) {
+  if (!S || !hits || !misses) return ma::fail(MA_ERR_INVALID, "ma_server_plan_cache: null argument");
+  S->plans.counts(hits, misses);
   return MA_OK;
 }
 

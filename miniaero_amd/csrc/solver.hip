@@ -63,6 +63,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -1790,22 +1791,23 @@ void plan_stats(const Layout& L, long long* stats) {
 
 // Device arrays of one run from its plan: the layout, the per-case
 // parameters and the work buffers.  With s->counting set nothing is
-// allocated or copied and the plan is left intact; s->arena_used is then the
-// device bytes the run needs.  Otherwise the plan's arrays are consumed.
-ma_status build_planned(ma_solver* s, RunPlan& P) {
+// allocated or copied; s->arena_used is then the device bytes the run needs.
+// The plan is only read, so one plan can build many runs (the served form
+// keeps plans of the mesh shapes it has seen).
+ma_status build_planned(ma_solver* s, const RunPlan& P) {
   ma_status st = MA_OK;
   const int ncases = s->ncases;
   s->all_lattice = P.all_lattice;
-  Layout& L = P.L;
-  std::vector<int>& tile_case = P.tile_case;
-  std::vector<int>& case_tiles = P.case_tiles;
-  std::vector<int>& bc_case = P.bc_case;
-  std::vector<int>& cell_case = P.cell_case;
+  const Layout& L = P.L;
+  const std::vector<int>& tile_case = P.tile_case;
+  const std::vector<int>& case_tiles = P.case_tiles;
+  const std::vector<int>& bc_case = P.bc_case;
+  const std::vector<int>& cell_case = P.cell_case;
   const int nc = L.nc, nif = L.nif, nf = L.nf;
   const long long nbf = L.nbf;
   if (!s->counting) {
-    s->case_cell_off.swap(P.case_cell_off);
-    s->dev_of.swap(L.dev_of);
+    s->case_cell_off = P.case_cell_off;
+    s->dev_of = L.dev_of;
   }
   DevMesh& m = s->m;
   m.nc = nc;
@@ -1904,7 +1906,6 @@ ma_status build_planned(ma_solver* s, RunPlan& P) {
   s->t.slot = dtslot;
   s->t.cptr = dtcptr;
   s->t.tcase = dtcase;
-  if (!s->counting) L = Layout();
 
   // per-case parameters
   std::vector<double> dt0(ncases), cfl(ncases), zero(ncases, 0.0);
@@ -2726,36 +2727,70 @@ extern "C" void ma_solver_destroy(ma_solver* s) {
 
 namespace ma {
 
-struct PlannedCase {
+// The device layout of one mesh, shareable between the runs of that mesh.
+struct LayoutPlan {
   RunPlan P;
-  ma_options opt;
   TileShape shape;
+  size_t host_bytes = 0;
+};
+
+struct PlannedCase {
+  std::shared_ptr<const LayoutPlan> lp;
+  ma_options opt;
   size_t bytes = 0;
 };
 
-ma_status plan_case(const ma_mesh_data* md, const ma_options* opt, PlannedCase** out, size_t* bytes) {
-  *out = nullptr;
+namespace {
+template <class V>
+size_t vbytes(const V& v) {
+  return v.size() * sizeof(typename V::value_type);
+}
+}  // namespace
+
+ma_status plan_layout_shared(const ma_mesh_data* md, const ma_options* opt, std::shared_ptr<const LayoutPlan>* out,
+                             long long* cells, size_t* host_bytes) {
   ma_status st = check_mesh(md, "serve");
   if (st || (st = check_options(opt))) return st;
   const TileShape shape = shape_of(*opt);
   if (!valid_shape(shape)) return ma::fail(MA_ERR_INVALID, "tile shape: extents >= 1 with product <= 256");
+  try {
+    std::shared_ptr<LayoutPlan> lp = std::make_shared<LayoutPlan>();
+    lp->shape = shape;
+    if ((st = plan_run(1, &md, shape, lp->P))) return st;
+    const Layout& L = lp->P.L;
+    lp->host_bytes = vbytes(L.dev_of) + vbytes(L.org_of) + vbytes(L.tile_cptr) + vbytes(L.tptr) +
+                     vbytes(L.tfaces) + vbytes(L.slot) + vbytes(L.tslot) + vbytes(L.fl) + vbytes(L.fr) +
+                     vbytes(L.fn) + vbytes(L.ffr) + vbytes(L.fx) + vbytes(L.fwall) + vbytes(L.vol) +
+                     vbytes(L.xc) + vbytes(L.fkind) + vbytes(lp->P.tile_case) + vbytes(lp->P.case_tiles) +
+                     vbytes(lp->P.bc_case) + vbytes(lp->P.cell_case) + vbytes(lp->P.case_cell_off);
+    *cells = L.nc;
+    *host_bytes = lp->host_bytes;
+    *out = std::move(lp);
+  } catch (const std::bad_alloc&) {
+    return ma::fail(MA_ERR_NOMEM, "serve: out of host memory");
+  }
+  return MA_OK;
+}
+
+ma_status plan_case_with(const std::shared_ptr<const LayoutPlan>& lp, const ma_options* opt, PlannedCase** out,
+                         size_t* bytes) {
+  *out = nullptr;
+  ma_status st = check_options(opt);
+  if (st) return st;
   PlannedCase* pc = new (std::nothrow) PlannedCase();
   if (!pc) return ma::fail(MA_ERR_NOMEM, "serve: out of host memory");
   try {
+    pc->lp = lp;
     pc->opt = *opt;
-    pc->shape = shape;
-    st = plan_run(1, &md, shape, pc->P);
-    if (!st) {
-      ma_solver count;  // sizes only: no device memory, no stream
-      count.counting = true;
-      count.opt = *opt;
-      count.ncases = 1;
-      count.case_opt.assign(1, *opt);
-      count.fp32 = opt->precision == MA_PRECISION_FP32;
-      count.shape = shape;
-      st = build_planned(&count, pc->P);
-      pc->bytes = count.arena_used;
-    }
+    ma_solver count;  // sizes only: no device memory, no stream
+    count.counting = true;
+    count.opt = *opt;
+    count.ncases = 1;
+    count.case_opt.assign(1, *opt);
+    count.fp32 = opt->precision == MA_PRECISION_FP32;
+    count.shape = lp->shape;
+    st = build_planned(&count, lp->P);
+    pc->bytes = count.arena_used;
   } catch (const std::bad_alloc&) {
     st = ma::fail(MA_ERR_NOMEM, "serve: out of host memory");
   }
@@ -2768,6 +2803,15 @@ ma_status plan_case(const ma_mesh_data* md, const ma_options* opt, PlannedCase**
   return MA_OK;
 }
 
+ma_status plan_case(const ma_mesh_data* md, const ma_options* opt, PlannedCase** out, size_t* bytes) {
+  *out = nullptr;
+  std::shared_ptr<const LayoutPlan> lp;
+  long long cells = 0;
+  size_t hb = 0;
+  ma_status st = plan_layout_shared(md, opt, &lp, &cells, &hb);
+  return st ? st : plan_case_with(lp, opt, out, bytes);
+}
+
 void destroy_plan(PlannedCase* pc) { delete pc; }
 
 ma_status create_from_plan(PlannedCase* pc, void* region, size_t region_bytes, ma_solver** out) {
@@ -2776,7 +2820,7 @@ ma_status create_from_plan(PlannedCase* pc, void* region, size_t region_bytes, m
   if (!s) return ma::fail(MA_ERR_NOMEM, "serve: out of host memory");
   ma_status st = MA_OK;
   try {
-    s->shape = pc->shape;
+    s->shape = pc->lp->shape;
     s->wg_per_cu = pc->opt.workgroups_per_cu;
     s->opt = pc->opt;
     s->ncases = 1;
@@ -2789,7 +2833,7 @@ ma_status create_from_plan(PlannedCase* pc, void* region, size_t region_bytes, m
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
       st = ma::fail(MA_ERR_DEVICE, "hipStreamCreate failed");
     else
-      st = build_planned(s, pc->P);
+      st = build_planned(s, pc->lp->P);
   } catch (const std::bad_alloc&) {
     st = ma::fail(MA_ERR_NOMEM, "serve: out of host memory");
   }

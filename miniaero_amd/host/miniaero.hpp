@@ -233,8 +233,8 @@ class BatchSolver {
 class Server {
  public:
   explicit Server(long long hbm_budget_bytes = 0, int max_active = 0, int setup_threads = 0,
-                  int steps_in_flight = 0) {
-    ma_server_config cfg{hbm_budget_bytes, max_active, setup_threads, steps_in_flight};
+                  int steps_in_flight = 0, long long plan_cache_bytes = 0) {
+    ma_server_config cfg{hbm_budget_bytes, max_active, setup_threads, steps_in_flight, plan_cache_bytes};
     check(ma_server_create(&cfg, &s_));
   }
   ~Server() { ma_server_destroy(s_); }

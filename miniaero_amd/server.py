@@ -24,10 +24,10 @@ from .options import Options
 
 class Server:
     def __init__(self, hbm_budget_bytes: float = 0, max_active: int = 0, setup_threads: int = 0,
-                 steps_in_flight: int = 0):
+                 steps_in_flight: int = 0, plan_cache_bytes: float = 0):
         self._L = _lib.load()
         cfg = _lib.ma_server_config(int(hbm_budget_bytes), int(max_active), int(setup_threads),
-                                    int(steps_in_flight))
+                                    int(steps_in_flight), int(plan_cache_bytes))
         h = ctypes.c_void_p()
         _lib.check(self._L.ma_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
@@ -59,6 +59,12 @@ class Server:
         _lib.check(self._L.ma_server_stats(self._h, s))
         return dict(submitted=s[0], waiting=s[1], active=s[2], finished=s[3], region_used=s[4],
                     region_size=s[5])
+
+    def plan_cache(self) -> dict:
+        """Requests that reused a cached mesh layout, and those that planned one."""
+        h, m = ctypes.c_longlong(), ctypes.c_longlong()
+        _lib.check(self._L.ma_server_plan_cache(self._h, ctypes.byref(h), ctypes.byref(m)))
+        return dict(hits=h.value, misses=m.value)
 
     def now(self) -> float:
         return float(self._L.ma_server_now(self._h))

@@ -3,11 +3,12 @@ advanced by the same launches (continuous batching).
 
 Bar: every case that ran to its end equals, bit for bit, the same case run
 alone with TimeSolverExplicitRK4 (itself held bit for bit against the oracle
-by the parity tests), whatever cases it shared launches with and whenever it
-joined; requests are admitted in submission order with at most max_active on
-the device; cases that cannot run (bad options, larger than the server's
-region) fail alone without disturbing the others; the region is empty again
-at the end.
+by the parity tests), whatever cases it shared launches with, whenever it
+joined and whether its mesh layout was planned for it or reused from an
+earlier request of the same mesh; requests are admitted in submission order
+with at most max_active on the device; cases that cannot run (bad options,
+larger than the server's region) fail alone without disturbing the others;
+the region is empty again at the end.
 """
 import time
 
@@ -66,9 +67,12 @@ def test_served_cases_match_their_solo_runs():
                 srv.release(r["id"])
         assert len(results) == len(ids), f"only {len(results)} of {len(ids)} finished"
         st = srv.stats()
+        pc = srv.plan_cache()
     finally:
         srv.close()
-    print(f"[gpu-server] stats {st}", flush=True)
+    print(f"[gpu-server] stats {st}, plan cache {pc}", flush=True)
+    # the last two requests repeat the meshes of the first two: their layouts come from the cache
+    assert pc["hits"] >= 2 and pc["hits"] + pc["misses"] == len(ids)
     assert st["finished"] == len(ids) and st["active"] == 0 and st["waiting"] == 0
     assert st["region_used"] == 0
     by_opts = {i: o for i, o in ids.items()}
