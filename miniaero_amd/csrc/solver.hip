@@ -36,12 +36,15 @@
 // results equal those of a one-thread CPU run of the reference.
 //
 // Layout in HBM (structure of arrays, one plane per component, so a
-// wavefront's 64 lanes read 64 consecutive doubles).  Cells are renumbered
-// into 4 x 8 x 8 bricks of the mesh lattice (one brick per flux tile) when the
-// mesh is a lattice; results go back to the caller in the caller's order.
+// wavefront's 64 lanes read 64 consecutive doubles; the gradient records,
+// gathered per face by the flux kernel, are one 16-byte-aligned block per
+// cell instead).  Cells are renumbered into 4 x 8 x 8 bricks of the mesh
+// lattice (one brick per flux tile) when the mesh is a lattice; results go
+// back to the caller in the caller's order.
 //   cells   vol[nc], U[2][5][nc] (U_n / U_acc ping-pong), R[5][nc],
-//           V[2][5][nc] (stage primitives, even/odd stages), grad[15][nc],
-//           phi[5][nc]; int32 slot[6][nc] (face*2 + side, reference order),
+//           V[2][5][nc] (stage primitives, even/odd stages),
+//           rec[nc][20] (gradient 15, limiter 5 per cell);
+//           int32 slot[6][nc] (face*2 + side, reference order),
 //           nbr[6][nc] (the cell across each slot), tile slot codes [6][nc];
 //           xc[3][nc] for the initial condition only
 //   faces   n[3][nf] (area vector; an FP32 copy in the FP32 form),
@@ -137,6 +140,71 @@ template <class S, class T>
 __device__ __forceinline__ void load3(const S* __restrict__ a, int n, int i, T* v) {
 #pragma unroll
   for (int k = 0; k < 3; ++k) v[k] = T(a[(size_t)k * n + i]);
+}
+
+// Gradient record: the gradient (15 values, component k and direction d at
+// 3k + d) and the limiter (5 values) of a cell stored together, kRec values
+// per cell, cell after cell.  The flux kernel gathers both for the two cells
+// of every face; as one 16-byte-aligned block per cell a gather moves only the
+// cell's own bytes, where 20 separate planes would each bring in a cache
+// line mostly of other cells (a tile's neighbours in a brick lie every 8th or
+// 64th cell).
+constexpr int kRec = 20;
+
+template <class T>
+__device__ __forceinline__ void load_rec(const T* __restrict__ rec, int c, T* v) {
+  if constexpr (sizeof(T) == 8) {
+    const double2* p = reinterpret_cast<const double2*>(rec + (size_t)c * kRec);
+#pragma unroll
+    for (int i = 0; i < kRec / 2; ++i) {
+      const double2 x = p[i];
+      v[2 * i] = x.x;
+      v[2 * i + 1] = x.y;
+    }
+  } else {
+    const float4* p = reinterpret_cast<const float4*>(rec + (size_t)c * kRec);
+#pragma unroll
+    for (int i = 0; i < kRec / 4; ++i) {
+      const float4 x = p[i];
+      v[4 * i] = x.x;
+      v[4 * i + 1] = x.y;
+      v[4 * i + 2] = x.z;
+      v[4 * i + 3] = x.w;
+    }
+  }
+}
+
+// Shared memory that stores the gradient records of a workgroup of kBlock
+// cells: a row of kRec + 1 values per thread (the pad spreads the rows over
+// the LDS banks).
+template <class T>
+constexpr int kStageRow = kRec + 1;
+
+// Each thread puts its cell's record (out) in its row, then every wavefront
+// writes its 64 cells' records -- one contiguous block in rec, from its first
+// cell c0 on -- 16 bytes per lane per store, so the stores are whole cache
+// lines instead of each lane writing its own record.  Every thread of the
+// workgroup calls it (it holds a barrier); cells at or beyond nc are not
+// written.
+template <class T>
+__device__ __forceinline__ void store_records(T* __restrict__ rec, T* stage, const T* out, int c0, int nc) {
+  T* row = stage + threadIdx.x * kStageRow<T>;
+#pragma unroll
+  for (int j = 0; j < kRec; ++j) row[j] = out[j];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int first = c0 + w * 64;
+  const int nvalid = min(64, nc - first);
+  constexpr int E = 16 / sizeof(T);  // values per 16-byte chunk; kRec is a multiple of it
+  const T* wst = stage + w * 64 * kStageRow<T>;
+  for (int e = lane * E; e < nvalid * kRec; e += 64 * E) {
+    const T* src = wst + (e / kRec) * kStageRow<T> + e % kRec;
+    T* dst = rec + (size_t)first * kRec + e;
+    if constexpr (sizeof(T) == 8)
+      *reinterpret_cast<double2*>(dst) = make_double2(src[0], src[1]);
+    else
+      *reinterpret_cast<float4*>(dst) = make_float4(src[0], src[1], src[2], src[3]);
+  }
 }
 
 // Area vectors in the arithmetic type.
@@ -306,10 +374,14 @@ __device__ __forceinline__ void load_disp(const DevMesh& m, int ent, T* d) {
 // extrapolation v + phi ((x_f - x_c) . grad) of Flux.h:107-130, the same
 // expression in the same order as the flux kernel's otherwise -- is stored
 // in fv[kMaxSlots * 5][nc], so the flux kernel gathers 5 values per face
-// side instead of 25; grad and phi are then stored only when non-null.
+// side instead of 25; the gradient record is then stored (to rec, kRec
+// values per cell) only when rec is non-null, by the getter's call.
+// Otherwise the record is left in out[kRec] for the caller, which stores a
+// wavefront's records together (store_records).  First order has zero
+// limiters.
 template <bool kSecondOrder, bool kFaceVals, class Limiter, class T>
 __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restrict__ nbr, const T* __restrict__ V,
-                                          T* __restrict__ grad, T* __restrict__ phi, T* __restrict__ fv, int c) {
+                                          T* __restrict__ rec, T* __restrict__ fv, int c, T* out) {
   const int nc = m.nc, nf = m.nf;
   int ent[kMaxSlots], nb[kMaxSlots];
 #pragma unroll
@@ -389,9 +461,19 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
       T x = 0;
       x += g[k][d];
       g[k][d] = x;
-      if (!kFaceVals || grad) grad[(size_t)(k * 3 + d) * nc + c] = x;
+      if constexpr (kFaceVals) {
+        if (rec) rec[(size_t)c * kRec + k * 3 + d] = x;  // the getter's call only
+      } else {
+        out[k * 3 + d] = x;
+      }
     }
-  if (!kSecondOrder) return;
+  if (!kSecondOrder) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      if constexpr (!kFaceVals) out[15 + k] = T(0);
+    }
+    return;
+  }
 
   T lim[5];
   T dus[kFaceVals ? kMaxSlots : 1][5];  // the extrapolation increments, kept for the face values
@@ -427,9 +509,13 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
       lim[k] = ok ? ma::smin(lim[k], l) : lim[k];
     }
   }
-  if (!kFaceVals || phi) {
 #pragma unroll
-    for (int k = 0; k < 5; ++k) phi[(size_t)k * nc + c] = lim[k];
+  for (int k = 0; k < 5; ++k) {
+    if constexpr (kFaceVals) {
+      if (rec) rec[(size_t)c * kRec + 15 + k] = lim[k];
+    } else {
+      out[15 + k] = lim[k];
+    }
   }
   if (kFaceVals) {
 #pragma unroll
@@ -450,13 +536,17 @@ constexpr int kGradWaves = sizeof(T) == 8 && kSecondOrder ? 3 : 1;
 
 template <bool kSecondOrder, bool kFaceVals, class Limiter, class T>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGradWaves<kSecondOrder, T>)))
-void k_grad(DevMesh m, const int* __restrict__ nbr, const T* __restrict__ V,
-                                                 T* __restrict__ grad, T* __restrict__ phi, T* __restrict__ fv,
-                                                 int per_xcd) {
+void k_grad(DevMesh m, const int* __restrict__ nbr, const T* __restrict__ V, T* __restrict__ rec,
+            T* __restrict__ fv, int per_xcd) {
+  __shared__ T stage[kFaceVals ? 1 : kBlock * kStageRow<T>];
   // XCD-aware: each XCD takes a contiguous run of blocks (see k_flux)
   const int b = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
   const int c = b * kBlock + threadIdx.x;
-  if (c < m.nc) grad_cell<kSecondOrder, kFaceVals, Limiter, T>(m, nbr, V, grad, phi, fv, c);
+  T out[kRec];
+  if (c < m.nc) grad_cell<kSecondOrder, kFaceVals, Limiter, T>(m, nbr, V, rec, fv, c, out);
+  if constexpr (!kFaceVals) {
+    if (b * kBlock < m.nc) store_records(rec, stage, out, b * kBlock, m.nc);  // uniform over the workgroup
+  }
 }
 
 // Flux of internal face f between cells L and Rr: compute_face_flux's body
@@ -471,8 +561,7 @@ void k_grad(DevMesh m, const int* __restrict__ nbr, const T* __restrict__ V,
 // every workgroup that evaluates face f gets the same bits.
 template <bool kSecondOrder, class Inv, class Visc, class T>
 __device__ __forceinline__ void interior_flux(const DevMesh& m, const T* __restrict__ V,
-                                              const T* __restrict__ grad,
-                                              const T* __restrict__ phi, const T* __restrict__ fv, int f,
+                                              const T* __restrict__ rec, const T* __restrict__ fv, int f,
                                               int L, int sL, int Rr, int sR, const Inv& inv, const Visc& visc,
                                               T* F) {
   constexpr bool kFaceVals = kSecondOrder && !Visc::isViscous;
@@ -489,16 +578,22 @@ __device__ __forceinline__ void interior_flux(const DevMesh& m, const T* __restr
     load5(V, nc, Rr, vr);
   }
   load3(area<T>(m), nf, f, a);
-  if (kGrad) {
+  if (kGrad) {  // (kRecon implies kGrad)
+    T ra[kRec], rb[kRec];
+    load_rec(rec, L, ra);
+    load_rec(rec, Rr, rb);
 #pragma unroll
     for (int j = 0; j < 15; ++j) {
-      gl[j] = grad[(size_t)j * nc + L];
-      gr[j] = grad[(size_t)j * nc + Rr];
+      gl[j] = ra[j];
+      gr[j] = rb[j];
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      pl[k] = ra[15 + k];
+      pr[k] = rb[15 + k];
     }
   }
   if (kRecon) {
-    load5(phi, nc, L, pl);
-    load5(phi, nc, Rr, pr);
     const T* dx = static_cast<const T*>(m.fdx);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -600,9 +695,8 @@ template <bool kSecondOrder, class Inv, class Visc, class T>
 // new state just written to Uacc_out (the next step's stage 0, alpha 0).
 // R itself is stored on the last stage only.
 __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, int tile, T* lds,
-                                          const T* __restrict__ V, const T* __restrict__ grad,
-                                          const T* __restrict__ phi, const T* __restrict__ fv,
-                                          const T* __restrict__ bc,
+                                          const T* __restrict__ V, const T* __restrict__ rec,
+                                          const T* __restrict__ fv, const T* __restrict__ bc,
                                           const double* Uacc_in, double* Uacc_out, double* __restrict__ R,
                                           const double* __restrict__ dt_dev, double beta, const Inv& inv,
                                           const Visc& visc, const double* __restrict__ Un,
@@ -627,7 +721,7 @@ __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, in
     T F[5];
     // slots are packed into the cell entries only for the face-value scheme
     constexpr bool kFaceVals = kSecondOrder && !Visc::isViscous;
-    interior_flux<kSecondOrder>(m, V, grad, phi, fv, fc, kFaceVals ? cell_of(lrc.x) : lrc.x, slot_of(lrc.x),
+    interior_flux<kSecondOrder>(m, V, rec, fv, fc, kFaceVals ? cell_of(lrc.x) : lrc.x, slot_of(lrc.x),
                                 kFaceVals ? cell_of(lrc.y) : lrc.y, slot_of(lrc.y), inv, visc, F);
 #pragma unroll
     for (int k = 0; k < 5; ++k) lds[k * mf + i] = F[k];
@@ -711,8 +805,7 @@ constexpr int kFluxWaves = sizeof(T) == 8 ? 3 : 1;
 template <bool kSecondOrder, class Inv, class Visc, class T>
 __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kFluxWaves<T>)))
 void k_flux(DevMesh m, TileMap t, const T* __restrict__ V,
-            const T* __restrict__ grad,
-            const T* __restrict__ phi,
+            const T* __restrict__ rec,  // gradient records, or the face values (inviscid second order)
             const T* __restrict__ bc,
             const double* Uacc_in,  // may alias Uacc_out
             double* Uacc_out, double* __restrict__ R,
@@ -730,9 +823,9 @@ void k_flux(DevMesh m, TileMap t, const T* __restrict__ V,
   for (int j = blockIdx.x / kXcds; j < t.per_xcd; j += stride) {
     const int tile = xcd * t.per_xcd + j;
     if (tile >= t.ntiles) break;  // uniform over the workgroup
-    // the face-value scheme reads no gradients: its face values come in grad
-    flux_tile<kSecondOrder, Inv, Visc, T>(m, t, tile, lds, V, grad, phi, grad, bc, Uacc_in, Uacc_out, R, dt_dev,
-                                          beta, inv, visc, Un, Vnext, alpha_next, last);
+    // the face-value scheme reads no gradients: its face values come in rec
+    flux_tile<kSecondOrder, Inv, Visc, T>(m, t, tile, lds, V, rec, rec, bc, Uacc_in, Uacc_out, R, dt_dev, beta,
+                                          inv, visc, Un, Vnext, alpha_next, last);
     __syncthreads();  // the LDS is reused by the next tile
   }
 }
@@ -898,7 +991,8 @@ struct CaseView {
   TileMap t;
   double* U[2];
   double* R;
-  void *V[2], *grad, *phi, *bc;  // arithmetic type of the launch; V[stage & 1] holds the stage's primitives
+  void *V[2], *grad, *bc;  // arithmetic type of the launch; V[stage & 1] holds the stage's primitives;
+                           // grad: gradient records
   void* fv;                      // extrapolated face values (inviscid second order)
   double *dt, *time, *partial;
   const double* cfl;
@@ -943,12 +1037,16 @@ void k_grad_multi(const CaseView* __restrict__ cv, const int* __restrict__ start
   if (g >= start[n]) return;
   const int k = find_case(start, n, g);
   const CaseView& v = cv[k];
-  const int c = (g - start[k]) * kBlock + threadIdx.x;
+  const int c0 = (g - start[k]) * kBlock, c = c0 + threadIdx.x;
+  T out[kRec];
   if (c < v.m.nc)
     grad_cell<kSecondOrder, kFaceVals, Limiter, T>(v.m, v.nbr, static_cast<const T*>(v.V[vsel]),
                                                    store_grad ? static_cast<T*>(v.grad) : nullptr,
-                                                   store_grad ? static_cast<T*>(v.phi) : nullptr,
-                                                   static_cast<T*>(v.fv), c);
+                                                   static_cast<T*>(v.fv), c, out);
+  if constexpr (!kFaceVals) {
+    __shared__ T stage[kBlock * kStageRow<T>];
+    if (store_grad) store_records(static_cast<T*>(v.grad), stage, out, c0, v.m.nc);  // uniform over the workgroup
+  }
 }
 
 template <class Inv, class T>
@@ -977,8 +1075,7 @@ void k_flux_multi(const CaseView* __restrict__ cv, const int* __restrict__ start
   double* Uacc = v.U[1 - cur];
   flux_tile<kSecondOrder, Inv, Visc, T>(v.m, v.t, g - start[k], reinterpret_cast<T*>(lds_raw),
                                    static_cast<const T*>(v.V[irk & 1]), static_cast<const T*>(v.grad),
-                                   static_cast<const T*>(v.phi), static_cast<const T*>(v.fv),
-                                   static_cast<const T*>(v.bc),
+                                   static_cast<const T*>(v.fv), static_cast<const T*>(v.bc),
                                    irk == 0 ? v.U[cur] : Uacc, Uacc, v.R, v.dt, beta, inv, visc, v.U[cur],
                                    static_cast<T*>(v.V[1 - (irk & 1)]), alpha_next, irk == 3);
 }
@@ -1044,7 +1141,8 @@ struct ma_solver {
   double *U[2] = {nullptr, nullptr}, *R = nullptr;
   // Arithmetic-type buffers (double, or float for the FP32 form): primitives,
   // gradients, limiters, boundary contributions.
-  void *V = nullptr, *V2 = nullptr, *grad = nullptr, *phi = nullptr, *bc = nullptr;
+  void *V = nullptr, *V2 = nullptr, *bc = nullptr;
+  void* grad = nullptr;  // gradient records (kRec values per cell: gradient, limiter)
   void* fv = nullptr;  // kMaxSlots x 5 planes: extrapolated face values (inviscid second order)
   bool fp32 = false;
   // Cases (independent meshes stepped together; a plain run is one case).
@@ -1941,8 +2039,7 @@ ma_status build_planned(ma_solver* s, const RunPlan& P) {
   const size_t tsize = s->fp32 ? sizeof(float) : sizeof(double);
   if ((st = s->alloc(&s->U[0], n5)) || (st = s->alloc(&s->U[1], n5)) || (st = s->alloc(&s->R, n5)) ||
       (st = s->alloc_bytes(&s->V, n5 * tsize)) || (st = s->alloc_bytes(&s->V2, n5 * tsize)) ||
-      (st = s->alloc_bytes(&s->grad, 3 * n5 * tsize)) ||
-      (st = s->alloc_bytes(&s->phi, n5 * tsize)) ||
+      (st = s->alloc_bytes(&s->grad, (size_t)kRec * nc * tsize)) ||
       (s->opt.second_order_space && !s->opt.viscous &&
        (st = s->alloc_bytes(&s->fv, (size_t)kMaxSlots * n5 * tsize))) ||
       (st = s->alloc_bytes(&s->bc, (size_t)5 * std::max<long long>(nbf, 1) * tsize)) ||
@@ -1952,8 +2049,7 @@ ma_status build_planned(ma_solver* s, const RunPlan& P) {
     return st;
   if (s->counting) return MA_OK;
   MA_HIP(hipMemsetAsync(s->R, 0, n5 * sizeof(double), s->stream));
-  MA_HIP(hipMemsetAsync(s->grad, 0, 3 * n5 * tsize, s->stream));
-  MA_HIP(hipMemsetAsync(s->phi, 0, n5 * tsize, s->stream));
+  MA_HIP(hipMemsetAsync(s->grad, 0, (size_t)kRec * nc * tsize, s->stream));
   MA_HIP(hipStreamSynchronize(s->stream));
   return MA_OK;
 }
@@ -2001,7 +2097,7 @@ void run_stage(ma_solver* s, int irk) {
       const bool store = !FV;
       hipLaunchKernelGGL((k_grad<SO, FV, Lim, T>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, m, s->nbr,
                          static_cast<const T*>(Vs), store ? static_cast<T*>(s->grad) : nullptr,
-                         store ? static_cast<T*>(s->phi) : nullptr, static_cast<T*>(s->fv), per);
+                         static_cast<T*>(s->fv), per);
     });
   }
   Visc visc{effective_prandtl(s->opt)};
@@ -2020,7 +2116,7 @@ void run_stage(ma_solver* s, int irk) {
   s->launch("k_flux", [&] {
     hipLaunchKernelGGL((k_flux<SO, Inv, Visc, T>), dim3(resident(s, s->t.per_xcd) * kXcds), dim3(kTile), lds,
                        s->stream, m, s->t, static_cast<const T*>(Vs), static_cast<const T*>(FV ? s->fv : s->grad),
-                       static_cast<const T*>(s->phi), static_cast<const T*>(s->bc), acc_in, Uacc, s->R,
+                       static_cast<const T*>(s->bc), acc_in, Uacc, s->R,
                        s->dt_dev, beta[irk], inv, visc, Un, Vn, irk < 3 ? alpha[irk + 1] : alpha[0],
                        irk == 3 ? 1 : 0);
   });
@@ -2156,6 +2252,24 @@ ma_status download_aos(ma_solver* s, const S* dsrc, int ncomp, double* host) {
   for (int c = 0; c < nc; ++c) {
     const int d = s->dev_of[c];
     for (int k = 0; k < ncomp; ++k) host[(size_t)c * ncomp + k] = tmp[(size_t)k * nc + d];
+  }
+  return MA_OK;
+}
+
+// Gradient records (kRec values per device cell) to the caller's order:
+// grad (n x 5 x 3) and lim (n x 5), either may be null.
+template <class S>
+ma_status download_records(ma_solver* s, const S* dsrc, double* grad, double* lim) {
+  const int nc = s->m.nc;
+  std::vector<S> tmp((size_t)kRec * nc);
+  MA_HIP(hipMemcpyAsync(tmp.data(), dsrc, tmp.size() * sizeof(S), hipMemcpyDeviceToHost, s->stream));
+  MA_HIP(hipStreamSynchronize(s->stream));
+  for (int c = 0; c < nc; ++c) {
+    const S* r = &tmp[(size_t)s->dev_of[c] * kRec];
+    if (grad)
+      for (int j = 0; j < 15; ++j) grad[(size_t)c * 15 + j] = r[j];
+    if (lim)
+      for (int k = 0; k < 5; ++k) lim[(size_t)c * 5 + k] = r[15 + k];
   }
   return MA_OK;
 }
@@ -2444,8 +2558,7 @@ void stage3_gradients(ma_solver* s) {
   const int nb = nblocks(s->m.nc);
   const int per = (nb + kXcds - 1) / kXcds;
   hipLaunchKernelGGL((k_grad<true, true, Lim, T>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, s->m, s->nbr,
-                     static_cast<const T*>(s->V2), static_cast<T*>(s->grad), static_cast<T*>(s->phi),
-                     static_cast<T*>(s->fv), per);
+                     static_cast<const T*>(s->V2), static_cast<T*>(s->grad), static_cast<T*>(s->fv), per);
 }
 
 // Inviscid second order stores no gradients or limiters while stepping.  The
@@ -2473,14 +2586,8 @@ extern "C" ma_status ma_solver_get_gradients(ma_solver* s, double* grad, double*
   if (!s) return ma::fail(MA_ERR_INVALID, "null solver");
   ma_status st = fill_gradients(s);
   if (st) return st;
-  if (s->fp32) {
-    if (grad && (st = download_aos(s, static_cast<const float*>(s->grad), 15, grad))) return st;
-    if (lim && (st = download_aos(s, static_cast<const float*>(s->phi), 5, lim))) return st;
-  } else {
-    if (grad && (st = download_aos(s, static_cast<const double*>(s->grad), 15, grad))) return st;
-    if (lim && (st = download_aos(s, static_cast<const double*>(s->phi), 5, lim))) return st;
-  }
-  return MA_OK;
+  return s->fp32 ? download_records(s, static_cast<const float*>(s->grad), grad, lim)
+                 : download_records(s, static_cast<const double*>(s->grad), grad, lim);
 }
 
 namespace {
@@ -2930,7 +3037,6 @@ ma_status multi_set(Multi* M, ma_solver* const* cases, const int* parity, int n)
     v.V[0] = s->V;
     v.V[1] = s->V2;
     v.grad = s->grad;
-    v.phi = s->phi;
     v.fv = s->fv;
     v.bc = s->bc;
     v.dt = s->dt_dev;

@@ -9,6 +9,15 @@ C4, the whole 512-case batch as the benchmark runs it (autotuned tile shape,
     neighbours, the tile shape or the graphs; the four smallest cases (8.8k
     cells) equal the oracle's 100-step run bit for bit.
 
+C4's FP32 arithmetic form against its FP64 run, all 512 cases after their
+100 steps (momentum components scaled by the momentum-vector norm):
+whole state, rho and rho E within 1e-5 in every case; the momentum within
+1e-5 in every case whose left and right pressures differ by more than 1%.
+In the few cases with nearly equal pressures the momentum is driven by
+pressure differences near 1e-3 p, where the FP32 rounding of rho and T
+(5e-8 p in every face pressure) is 1e-5 to 1e-4 of it; there the bound is
+5e-5 and the errors are printed.
+
 C5, the first 40 requests of the seeded served stream (4k-1.9M cells, 59-496
 steps, viscous and inviscid, FP32 arithmetic), replayed at their arrival
 times so that they join while others run:
@@ -106,6 +115,30 @@ def test_c4_whole_batch_fp64_bitwise():
         same = np.array_equal(g2[k], U)
         _log(f"[c4] case {k} ({o.nx}x{o.ny}x{o.nz}) vs oracle, 100 steps: bitwise={same}")
         assert same, f"case {k} differs from the oracle"
+
+
+@pytest.mark.slow
+def test_c4_whole_batch_fp32_errors():
+    cases = batched_c4()
+    meshes = generate_meshes(cases)
+    f64, _ = _batch_states(meshes, cases, 100, graph=2)
+    f32, _ = _batch_states(meshes, [o.replace(precision=1) for o in cases], 100, graph=2)
+    near, worst = [], (0.0, -1)
+    for k, o in enumerate(cases):
+        whole, per, _ = precision_errors(f32[k], f64[k])
+        assert not np.array_equal(f32[k], f64[k])
+        assert whole <= TOL_FP32 and per[0] <= TOL_FP32 and per[4] <= TOL_FP32, (k, whole, per)
+        pl, pr = o.ic[1], o.ic[3]
+        mom = max(per[1:4])
+        worst = max(worst, (max([whole] + list(per)), k))
+        if abs(pl - pr) > 0.01 * max(pl, pr):
+            assert mom <= TOL_FP32, f"case {k}: momentum error {mom:.2e} (pressures {pl:.4f} / {pr:.4f})"
+        else:
+            near.append(k)
+            _log(f"[c4] case {k}: pressures {pl:.4f} / {pr:.4f}, FP32 momentum error {mom:.2e}")
+            assert mom <= 5 * TOL_FP32
+    _log(f"[c4] FP32 vs FP64: worst case {worst[1]} at {worst[0]:.2e}; {len(near)} near-equal-pressure cases")
+    assert len(near) <= 16
 
 
 def _serve(stream, max_active=32, budget=200e9):
