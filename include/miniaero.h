@@ -237,11 +237,11 @@ ma_status ma_batch_get_case_time(ma_solver* s, int k, double* time, double* last
  * them together while new requests arrive (continuous batching): the cases
  * of one scheme (order, viscous, limiter, precision, Prandtl) are advanced
  * by the same launches, one RK4 step at a time; a case joins at the next step
- * after its admission and leaves after its last one.  Requests are planned on host threads (mesh
- * generated from the options as Parallel3DMesh does, layout, exact device
- * size) and admitted in submission order when a slot is free and their
- * device bytes fit in the server's region of HBM, which is allocated once at
- * creation.  Each case runs opts->ntimesteps RK4 steps from its initial
+ * after its admission and leaves after its last one.  Requests are planned
+ * on host threads (mesh generated from the options as Parallel3DMesh does,
+ * layout -- reused for a mesh seen before -- and exact device size) and
+ * admitted in submission order when a slot is free and their device bytes
+ * fit in the server's region of HBM, which is allocated once at creation.  Each case runs opts->ntimesteps RK4 steps from its initial
  * state; its result equals a ma_solver_create run of the same case alone,
  * bit for bit.  No reference counterpart (the reference runs one case per
  * process, Main.C:96-141). */
@@ -249,7 +249,7 @@ typedef struct ma_server ma_server;
 
 typedef struct {
   long long hbm_budget_bytes; /* device region for admitted cases; 0 = 200e9, capped by free HBM */
-  int max_active;             /* cases on the device at once; 0 = 16 */
+  int max_active;             /* cases on the device at once; 0 = 16, at most 64 */
   int setup_threads;          /* host threads planning requests; 0 = 4 */
   int steps_in_flight;        /* RK4 steps queued ahead per scheme group; 0 = 2 */
   long long plan_cache_bytes; /* host bytes of device layouts kept for reuse by requests of the same

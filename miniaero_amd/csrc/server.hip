@@ -555,6 +555,12 @@ void scheduler(ma_server* S) {
 
 }  // namespace
 
+// The most cases the server keeps on the device at once.  A stream with no
+// slot limit (about 1,400 cases admitted by free HBM alone) stopped completing
+// cases without a device error, for a cause not found; the largest limit run
+// since is 64.
+constexpr int kMaxActive = 64;
+
 extern "C" ma_status ma_server_create(const ma_server_config* cfg, ma_server** out) {
   if (!out) return ma::fail(MA_ERR_INVALID, "ma_server_create: null argument");
   *out = nullptr;
@@ -563,6 +569,9 @@ extern "C" ma_status ma_server_create(const ma_server_config* cfg, ma_server** o
     return ma::fail(MA_ERR_INVALID, "ma_server_create: negative setting");
   if (c.hbm_budget_bytes == 0) c.hbm_budget_bytes = 200000000000ll;
   if (c.max_active == 0) c.max_active = 16;
+  if (c.max_active > kMaxActive)
+    return ma::fail(MA_ERR_INVALID, "ma_server_create: max_active above " + std::to_string(kMaxActive) +
+                                        " is not supported");
   if (c.setup_threads == 0) c.setup_threads = 4;
   if (c.steps_in_flight == 0) c.steps_in_flight = 2;
   if (c.plan_cache_bytes == 0) c.plan_cache_bytes = 16ll << 30;

@@ -258,6 +258,9 @@ def test_server_checks_config_and_device_cleanly():
     with pytest.raises(M.MiniAeroError) as e:
         M.Server(max_active=-1)
     assert e.value.status == _lib.MA_ERR_INVALID
+    with pytest.raises(M.MiniAeroError) as e:
+        M.Server(max_active=65)                          # above the largest slot limit run
+    assert e.value.status == _lib.MA_ERR_INVALID
     if not os.path.exists("/dev/kfd"):
         with pytest.raises(M.MiniAeroError) as e:
             M.Server()
