@@ -75,7 +75,8 @@ def algorithmic_bytes(kernel: str, nc: int, nif: int, nbf: int, second_order: bo
     if kernel == "k_prim":
         return (80.0 + 5 * t) * nc          # U_n, R read; V written
     if kernel == "k_dt_partial":
-        return 48.0 * nc + 24.0 * F
+        # U_n, the cell's 5 summed face-geometry values and vol read per cell
+        return 88.0 * nc
     return 0.0
 
 

@@ -109,6 +109,9 @@ struct DevMesh {
   const int* __restrict__ fr;
   const signed char* __restrict__ fkind;
   const double* __restrict__ fwall;  // 1 / |x_f - x_c| of boundary faces
+  // CFL runs: 5 planes of nc, each cell's face geometry S_x, S_y, S_z, N1, N2
+  // (see dt_tile); null otherwise
+  const double* __restrict__ dtg;
 };
 
 struct Inflow {
@@ -830,11 +833,15 @@ void k_flux(DevMesh m, TileMap t, const T* __restrict__ V,
   }
 }
 
-// Stable time step for the CFL extension, per cell over its slot list:
-//   lc = sum_f (|u.n_f| + c |n_f|),  lv = sum_f |n_f|^2,
+// Stable time step for the CFL extension, per cell:
+//   lc = |u| S_x + |v| S_y + |w| S_z + c N1,  lv = N2,
 //   dt_c = vol / (lc + 4 kappa lv / vol),  kappa = max(4/3, gamma/Pr) mu / rho
 // (kappa = 0 for inviscid runs: convective plus viscous spectral radius),
-// reduced to the minimum per block.
+// with the cell's face geometry summed once at setup over its slot list:
+// S_d = sum_f |n_f,d|, N1 = sum_f |n_f|, N2 = sum_f |n_f|^2.  For faces
+// normal to an axis (every box mesh, stretched or not) |u| S_x + |v| S_y +
+// |w| S_z is sum_f |u.n_f| exactly; for others it bounds it from above (a
+// smaller, still stable step).  Reduced to the minimum per block.
 // One workgroup of kTile threads per tile (a tile never spans two cases);
 // red: kTile / 64 doubles of shared memory.
 __device__ __forceinline__ void dt_tile(const DevMesh& m, const TileMap& t, int tile,
@@ -843,35 +850,20 @@ __device__ __forceinline__ void dt_tile(const DevMesh& m, const TileMap& t, int 
   const int c = t.cptr[tile] + threadIdx.x;
   double dtc = 1e300;
   if (c < t.cptr[tile + 1]) {
-    const int nc = m.nc, nf = m.nf;
-    double u[5], v[5];
+    const int nc = m.nc;
+    double u[5], v[5], gm[5];
     load5(Un, nc, c, u);
+    load5(m.dtg, nc, c, gm);
     ma::primitives(u, v);
     const double snd = sqrt(1.4 * (v[0] * ma::kRgas * v[4]) / v[0]);
-    // every slot's area vector is loaded before the sums (empty slots read face 0)
-    int ent[kMaxSlots];
-    double an[kMaxSlots][3];
-#pragma unroll
-    for (int q = 0; q < kMaxSlots; ++q) ent[q] = m.slot[(size_t)q * nc + c];
-#pragma unroll
-    for (int q = 0; q < kMaxSlots; ++q) load3(m.fn, nf, ent[q] < 0 ? 0 : ent[q] >> 1, an[q]);
-    double lc = 0.0, lv = 0.0;
-#pragma unroll
-    for (int q = 0; q < kMaxSlots; ++q) {
-      if (ent[q] < 0) break;
-      const double* n = an[q];
-      const double n2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
-      const double nn = sqrt(n2);
-      lc = lc + (fabs(v[1] * n[0] + v[2] * n[1] + v[3] * n[2]) + snd * nn);
-      lv = lv + n2;
-    }
+    const double lc = fabs(v[1]) * gm[0] + fabs(v[2]) * gm[1] + fabs(v[3]) * gm[2] + snd * gm[3];
     const double vol = m.vol[c];
     double lam = lc;
     if (viscous) {
       const double kfac = ma::smax(4.0 / 3.0, 1.4 / prandtl);
       const double mu = 1.458e-6 * v[4] * sqrt(v[4]) / (v[4] + 110.4);
       const double kappa = kfac * mu / v[0];
-      lam = lam + 4.0 * kappa * lv / vol;
+      lam = lam + 4.0 * kappa * gm[4] / vol;
     }
     dtc = vol / lam;
   }
@@ -1087,6 +1079,7 @@ __global__ __launch_bounds__(kTile) void k_dt_partial_multi(const CaseView* __re
   const int g = blockIdx.x;
   const int k = find_case(start, n, g);
   const CaseView& v = cv[k];
+  if (!v.m.dtg) return;  // a fixed-dt case: no geometry, its partial minima are not read
   dt_tile(v.m, v.t, g - start[k], v.U[v.parity ^ parity], viscous, prandtl, v.partial, red);
 }
 
@@ -2028,6 +2021,40 @@ ma_status build_planned(ma_solver* s, const RunPlan& P) {
     p.perturb = o.perturb;
     p.seed = o.seed;
     p.cell0 = (s->counting ? P.case_cell_off : s->case_cell_off)[k];
+  }
+  // CFL runs: each cell's face geometry for the time-step bound (dt_tile),
+  // summed over its slot list in slot order
+  s->m.dtg = nullptr;
+  if (s->any_cfl) {
+    std::vector<double> g;
+    if (!s->counting) {
+      g.assign((size_t)5 * nc, 0.0);
+      for (int c = 0; c < nc; ++c) {
+        double sx = 0.0, sy = 0.0, sz = 0.0, n1 = 0.0, n2 = 0.0;
+        for (int q = 0; q < kMaxSlots; ++q) {
+          const int e = L.slot[(size_t)q * nc + c];
+          if (e < 0) break;
+          const int f = e >> 1;
+          const double a0 = L.fn[f], a1 = L.fn[(size_t)nf + f], a2 = L.fn[(size_t)2 * nf + f];
+          const double q2 = a0 * a0 + a1 * a1 + a2 * a2;
+          sx = sx + std::fabs(a0);
+          sy = sy + std::fabs(a1);
+          sz = sz + std::fabs(a2);
+          n1 = n1 + std::sqrt(q2);
+          n2 = n2 + q2;
+        }
+        g[c] = sx;
+        g[(size_t)nc + c] = sy;
+        g[(size_t)2 * nc + c] = sz;
+        g[(size_t)3 * nc + c] = n1;
+        g[(size_t)4 * nc + c] = n2;
+      }
+    } else {
+      g.resize(1);
+    }
+    double* d = nullptr;
+    if ((st = s->counting ? s->alloc(&d, (size_t)5 * nc) : upload(s, &d, g))) return st;
+    s->m.dtg = d;
   }
   if ((st = upload(s, &s->dt_dev, dt0)) || (st = upload(s, &s->cfl_dev, cfl)) ||
       (st = upload(s, &s->time_dev, zero)) || (st = upload(s, &s->ic_dev, ic)) ||

@@ -915,23 +915,26 @@ void bc_flux(orc_solver* s, const Faces& G, int kind) {
   }
 }
 
-// Stable dt for the CFL extension, faces in the Green-Gauss order:
-//   lc = sum_f (|u.n_f| + c |n_f|),  lv = sum_f |n_f|^2,
+// Stable dt for the CFL extension (no reference counterpart):
+//   lc = |u| S_x + |v| S_y + |w| S_z + c N1,  lv = N2,
 //   dt_c = vol / (lc + 4 kappa lv / vol),  kappa = max(4/3, gamma/Pr) mu / rho
 // (kappa = 0 for inviscid runs; the convective plus viscous spectral radius
-// of Blazek, Computational Fluid Dynamics, sec. 6.1.4), dt = cfl * min_c dt_c.
+// of Blazek, Computational Fluid Dynamics, sec. 6.1.4, with sum_f |u.n_f|
+// taken as sum_d |u_d| S_d, equal to it for faces normal to an axis and an
+// upper bound otherwise), dt = cfl * min_c dt_c.  S_d = sum_f |n_f,d|,
+// N1 = sum_f |n_f|, N2 = sum_f |n_f|^2 over the cell's faces in the
+// Green-Gauss order.
 double stable_dt(orc_solver* s) {
   const orc_mesh* m = s->m;
-  std::vector<double> lc((size_t)s->nc, 0.0), lv((size_t)s->nc, 0.0);
-  std::vector<double> V((size_t)s->nc * 5);
-  for (int c = 0; c < s->nc; ++c) prim_of(s->sol_n, c, &V[(size_t)c * 5]);
+  std::vector<double> geo((size_t)s->nc * 5, 0.0);
   auto add = [&](int c, const double* n) {
-    const double* p = &V[(size_t)c * 5];
-    const double snd = std::sqrt(1.4 * (p[0] * kRgas * p[4]) / p[0]);
+    double* g = &geo[(size_t)c * 5];
     const double n2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
-    const double nn = std::sqrt(n2);
-    lc[c] = lc[c] + (std::fabs(p[1] * n[0] + p[2] * n[1] + p[3] * n[2]) + snd * nn);
-    lv[c] = lv[c] + n2;
+    g[0] = g[0] + std::fabs(n[0]);
+    g[1] = g[1] + std::fabs(n[1]);
+    g[2] = g[2] + std::fabs(n[2]);
+    g[3] = g[3] + std::sqrt(n2);
+    g[4] = g[4] + n2;
   };
   const Faces& F = m->internal;
   for (int ii = 0; ii < F.n; ++ii) {
@@ -941,6 +944,15 @@ double stable_dt(orc_solver* s) {
   }
   for (const Faces& G : m->groups)
     for (int i = 0; i < G.n; ++i) add(G.conn[2 * i], &G.a[3 * i]);
+  std::vector<double> V((size_t)s->nc * 5), lc((size_t)s->nc), lv((size_t)s->nc);
+  for (int c = 0; c < s->nc; ++c) {
+    double* p = &V[(size_t)c * 5];
+    prim_of(s->sol_n, c, p);
+    const double* g = &geo[(size_t)c * 5];
+    const double snd = std::sqrt(1.4 * (p[0] * kRgas * p[4]) / p[0]);
+    lc[c] = std::fabs(p[1]) * g[0] + std::fabs(p[2]) * g[1] + std::fabs(p[3]) * g[2] + snd * g[3];
+    lv[c] = g[4];
+  }
   const double Pr = s->o.prandtl > 0.0 ? s->o.prandtl : 0.71;
   const double kfac = std::max(4.0 / 3.0, 1.4 / Pr);
   double dtmin = 1e300;

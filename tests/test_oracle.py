@@ -288,3 +288,46 @@ def test_cfl_time_step_closed_form(viscous):
         lam += 4.0 * max(4.0 / 3.0, g / 0.72) * mu / rho * lv / vol
     expect = 0.5 * vol / lam
     assert s.time == pytest.approx(expect, rel=1e-12)
+
+
+@pytest.mark.parametrize("viscous", [0, 1])
+def test_cfl_time_step_bracket_on_a_skewed_mesh(viscous):
+    """On a ramp mesh (faces not normal to an axis) the CFL extension's
+    convective term sum_d |u_d| S_d, S_d = sum_f |n_f,d|, lies between the
+    spectral radius sum_f |u.n_f| and |u| sum_f |n_f| face by face
+    (|u.n| <= sum_d |u_d n_d| <= |u||n|), so the step is never larger than the
+    exact spectral-radius bound and never smaller than the isotropic one; on a
+    box the three coincide (closed form above)."""
+    rho, u, v, w, T = 0.9, 150.0, 60.0, -25.0, 280.0
+    Rg, g = 287.05, 1.4
+    E = rho * (Rg / (g - 1.0)) * T + 0.5 * rho * (u * u + v * v + w * w)
+    kw = dict(problem_type=1, lx=1.0, ly=0.5, lz=0.3, nx=9, ny=6, nz=4, angle=20.0, ntimesteps=1, dt=0.0,
+              second_order_space=0, viscous=viscous, bc_kind=(1, 1, 1, 1, 1, 1), use_inflow=1,
+              inflow=(rho, rho * u, rho * v, rho * w, E), prandtl=0.72, cfl=0.5)
+    m = O.OracleMesh(O.make_options(**kw))
+    s = O.OracleSolver(m)
+    s.init()
+    s.step(1)
+    nc = m.ncells
+    vel = np.array([u, v, w])
+    c = math.sqrt(g * Rg * T)
+    exact, iso, lv = np.zeros(nc), np.zeros(nc), np.zeros(nc)
+    faces = [(m.internal["conn"][:, 0], m.internal["a"]), (m.internal["conn"][:, 1], m.internal["a"])]
+    faces += [(G["conn"][:, 0], G["a"]) for G in m.groups]
+    skewed = 0
+    for cells, a in faces:
+        an = np.linalg.norm(a, axis=1)
+        skewed += int(np.sum(np.count_nonzero(np.abs(a) > 1e-12 * an[:, None], axis=1) > 1))
+        np.add.at(exact, cells, np.abs(a @ vel) + c * an)
+        np.add.at(iso, cells, (np.linalg.norm(vel) + c) * an)
+        np.add.at(lv, cells, an * an)
+    assert skewed > 0
+    vol = m.cell_vol
+    visc = 0.0
+    if viscous:
+        mu = 1.458e-6 * T * math.sqrt(T) / (T + 110.4)
+        visc = 4.0 * max(4.0 / 3.0, g / 0.72) * mu / rho * lv / vol
+    dt_exact = 0.5 * np.min(vol / (exact + visc))
+    dt_iso = 0.5 * np.min(vol / (iso + visc))
+    assert dt_iso * (1 - 1e-12) <= s.time <= dt_exact * (1 + 1e-12)
+    assert s.time < dt_exact * (1 - 1e-6)      # the ramp's faces make the bound strict
