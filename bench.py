@@ -226,8 +226,11 @@ def precision_errors(U, R) -> dict:
             **{n: float(np.linalg.norm(U[:, k] - R[:, k]) / scale[k]) for k, n in enumerate(names)}}
 
 
-def timed_run(solver, steps: int, warmup: int, graph: int = 0):
+def timed_run(solver, steps: int, warmup: int, graph: int = 0, persistent: bool = False):
     solver.initialize()
+    if persistent:
+        solver.set_persistent(True)
+        graph = 0
     if graph:
         solver.set_graph(graph)
     solver.step(warmup)
@@ -276,7 +279,7 @@ def bench_single(args, M, opts, copy_gbs) -> dict:
     shape, wg, tuned_ms = solver.tuning()
     print(f"[bench] tile shape {shape}, workgroups per CU {wg or 'all tiles'}"
           + (f" (autotuned, {tuned_ms:.3f} ms/step/Mcell)" if tuned_ms else ""), file=sys.stderr, flush=True)
-    elapsed = timed_run(solver, args.steps, args.warmup, args.graph)
+    elapsed = timed_run(solver, args.steps, args.warmup, args.graph, args.persistent)
     if solver.nonfinite():
         raise RuntimeError("non-finite state after the timed steps")
     print(f"[bench] {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
@@ -321,7 +324,9 @@ def bench_single(args, M, opts, copy_gbs) -> dict:
               "precision": "fp64" if opts.precision == 0 else "fp32 flux/gradient/limiter, fp64 state",
               "tile_shape": list(shape), "workgroups_per_cu": wg,
               "autotuned_ms_per_step_per_Mcell": tuned_ms or None,
-              "graph_steps": args.graph}
+              "graph_steps": 0 if args.persistent else args.graph,
+              "step_launch": "persistent (one cooperative launch per step() call)" if args.persistent
+              else "per kernel"}
     return dict(value=nc * 4 * args.steps / elapsed, elapsed=elapsed, config=config, roofline=roof,
                 cpu_baseline=cpu, fp32=opts.precision != 0, extra=extra)
 
@@ -342,7 +347,7 @@ def bench_batch(args, M, cases, copy_gbs) -> dict:
     shape, wg, tuned_ms = solver.tuning()
     print(f"[bench] tile shape {shape}, workgroups per CU {wg or 'all tiles'}"
           + (f" (autotuned, {tuned_ms:.3f} ms/step/Mcell)" if tuned_ms else ""), file=sys.stderr, flush=True)
-    elapsed = timed_run(solver, args.steps, args.warmup, args.graph)
+    elapsed = timed_run(solver, args.steps, args.warmup, args.graph, args.persistent)
     if solver.nonfinite():
         raise RuntimeError("non-finite state after the timed steps")
     print(f"[bench] {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
@@ -393,7 +398,9 @@ def bench_batch(args, M, cases, copy_gbs) -> dict:
               "precision": "fp32 flux/gradient/limiter, fp64 state" if fp32 else "fp64",
               "tile_shape": list(shape), "workgroups_per_cu": wg,
               "autotuned_ms_per_step_per_Mcell": tuned_ms or None,
-              "graph_steps": args.graph,
+              "graph_steps": 0 if args.persistent else args.graph,
+              "step_launch": "persistent (one cooperative launch per step() call)" if args.persistent
+              else "per kernel",
               "case_latency_ms": {"p50": round(per_case_ms, 2), "p99": round(per_case_ms, 2),
                                   "note": "all cases run in the same launches and finish together"}}
     return dict(value=nc * 4 * args.steps / elapsed, elapsed=elapsed, config=config, roofline=roof,
@@ -659,6 +666,9 @@ def main() -> int:
     ap.add_argument("--graph", type=int, default=None,
                     help="steps per captured HIP graph (even; 0 = plain launches); default 2 for "
                          "batched workloads, 0 otherwise")
+    ap.add_argument("--persistent", action="store_true",
+                    help="run the timed steps as one cooperative launch with grid-wide barriers "
+                         "between phases instead of a launch per kernel (graphs off)")
     ap.add_argument("--autotune", type=int, default=None,
                     help="1: pick the flux tile shape by timing candidates at setup (default: the "
                          "workload's setting)")

@@ -71,7 +71,8 @@ typedef struct ma_options {
   int use_inflow;            /* 1: inflow[] replaces the hard-coded inflow state */
   double inflow[5];          /* conserved inflow / freestream state */
   double prandtl;            /* 0 = 0.71 (GasModel.h:103) */
-  double cfl;                /* > 0: dt = cfl * min_c vol_c / sum_f(|u.n| + c|n|), per step */
+  double cfl;                /* > 0: dt = cfl * min_c vol_c / (|u| S_x + |v| S_y + |w| S_z + c sum_f |n_f|
+                              * + viscous term), S_d = sum_f |n_f,d| over the cell's faces, per step */
   double perturb;            /* MA_IC_PERTURBED_INFLOW: relative amplitude */
   unsigned long long seed;   /* MA_IC_PERTURBED_INFLOW: generator key */
   int precision;             /* MA_PRECISION_* */
@@ -169,6 +170,18 @@ ma_status ma_solver_synchronize(ma_solver* s);
  * solver: a step call from another thread that would use one fails with
  * MA_ERR_UNSUPPORTED and runs nothing.  No reference counterpart. */
 ma_status ma_solver_set_graph(ma_solver* s, int steps_per_graph);
+/* Persistent steps: with enabled = 1, ma_solver_step runs all its steps, for
+ * every case of the run, in one cooperative launch whose workgroups stay
+ * resident and pass a grid-wide barrier between the phases that are separate
+ * kernels otherwise (the time step, the gradients, the boundary faces, the
+ * flux tiles of each stage); same results as the per-kernel launches.  The
+ * call then waits for the device to be idle (every stream of the process)
+ * before the launch, and for the launch.  Turns graphs off (ma_solver_set_graph
+ * with a count > 0 turns this off).  MA_ERR_UNSUPPORTED when the device has
+ * no cooperative launch; MA_ERR_DEVICE when a barrier wait ran over its 10 s
+ * limit.  No reference counterpart: the reference launches each kernel of
+ * the stage loop separately (TimeSolverExplicitRK4.h:352-486). */
+ma_status ma_solver_set_persistent(ma_solver* s, int enabled);
 /* Launch shape in use: tile shape (-1 -1 -1 for consecutive cells),
  * resident workgroups per CU (0 = one per tile) and, after autotuning, the
  * measured ms per step per million cells of that choice (else 0).  Any

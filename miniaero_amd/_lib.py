@@ -32,7 +32,7 @@ EXPORTS = (
     "ma_solver_get_gradients", "ma_solver_get_time", "ma_solver_check_finite",
     "ma_solver_stream", "ma_solver_destroy", "ma_solver_profile", "ma_solver_sizes",
     "ma_measure_copy", "ma_mesh_plan", "ma_batch_create", "ma_batch_sizes", "ma_batch_get_case_state",
-    "ma_batch_get_case_time", "ma_batch_plan", "ma_solver_set_graph",
+    "ma_batch_get_case_time", "ma_batch_plan", "ma_solver_set_graph", "ma_solver_set_persistent",
     "ma_solver_tuning", "ma_solver_residual_norms", "ma_solver_residual_norms_noalloc", "ma_server_create", "ma_server_submit", "ma_server_poll",
     "ma_server_get_state", "ma_server_release", "ma_server_stats", "ma_server_now", "ma_server_destroy",
     "ma_server_plan_cache",
@@ -160,6 +160,7 @@ def load() -> ctypes.CDLL:
     L.ma_batch_plan.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.POINTER(ma_mesh_data)),
                                 ctypes.POINTER(ctypes.c_longlong)]
     L.ma_solver_set_graph.argtypes = [P, ctypes.c_int]
+    L.ma_solver_set_persistent.argtypes = [P, ctypes.c_int]
     L.ma_solver_tuning.argtypes = [P, _I, _I, _D]
     L.ma_solver_residual_norms.argtypes = [P, _D]
     L.ma_solver_residual_norms_noalloc.argtypes = [P, P]

@@ -1089,6 +1089,163 @@ __global__ __launch_bounds__(kBlock) void k_dt_final_multi(const CaseView* __res
   dt_case(v.partial, v.case_tiles, v.cfl, v.dt, v.time, 0, red);
 }
 
+// ------------------------------------------------------ persistent steps --
+// nsteps whole RK4 steps of every case of a run in one cooperative launch:
+// the workgroups stay resident and walk each phase's tiles or cell blocks
+// (the same device functions, in the same order, as the per-kernel launches),
+// with a grid-wide barrier between phases where the launches had a kernel
+// boundary.  Same bits as the per-kernel launches.
+
+struct PersistArgs {
+  DevMesh m;
+  TileMap t;
+  const int* nbr;
+  void* V[2];  // stage primitives: V (stages 0, 2), V2 (stages 1, 3)
+  void* grad;  // gradient records
+  void* fv;    // face values (inviscid second order)
+  void* bc;
+  double* U[2];
+  double* R;
+  double* dt;
+  double* time;
+  const double* cfl;
+  double* partial;
+  const int* case_tiles;
+  const Inflow* inflow;
+  const int* bc_case;
+  unsigned* gsync;  // [0] arrivals, [1] generation, [2] wait exceeded; zero at launch
+  long long wait_ticks;  // longest barrier wait, in wall-clock ticks
+  int ncases, nsteps, cur0, prim0, any_cfl, viscous, per_blocks;
+  double prandtl;
+};
+
+// Grid-wide barrier: every workgroup of the launch arrives once per call.
+// The arrivals are counted in gsync[0]; the last one resets the count and
+// advances the generation gsync[1], for which the others wait.  The fences
+// make each workgroup's earlier writes visible to the whole device (every
+// XCD's L2) before its arrival, and the new generation's observers see them.
+// The wait is bounded: after wait_ticks of the wall clock gsync[2] is set,
+// and a workgroup that finds it set leaves the kernel at this or any later
+// barrier, so the launch ends.  Returns false when the workgroup must leave.
+// Every thread of the workgroup calls it (it holds barriers).
+__device__ __forceinline__ bool grid_sync(unsigned* gsync, unsigned& gen, long long wait_ticks, int* stop) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned want = gen + 1;
+    int leave = 0;
+    __threadfence();
+    if (atomicAdd(&gsync[0], 1u) == gridDim.x - 1) {
+      atomicExch(&gsync[0], 0u);
+      __threadfence();
+      atomicExch(&gsync[1], want);
+    } else {
+      const long long t0 = wall_clock64();
+      while (__hip_atomic_load(&gsync[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != want) {
+        if (wall_clock64() - t0 > wait_ticks) {
+          atomicOr(&gsync[2], 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __threadfence();
+    if (__hip_atomic_load(&gsync[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) leave = 1;
+    gen = want;
+    *stop = leave;
+  }
+  __syncthreads();
+  return *stop == 0;
+}
+
+// Register budget of k_steps: the gradient and flux phases share one
+// allocation, which in FP64 does not fit the 3-wave budget of the separate
+// kernels without spilling, so FP64 runs 2 waves per SIMD (1 for the viscous
+// second-order form, whose spills then go to the accumulation registers) with
+// no scratch: the launch is refused if the kernel uses any, since scratch
+// allocations can hold fewer waves resident than the occupancy the grid is
+// sized for.
+template <bool kViscousSO, class T>
+constexpr int kStepsWaves = sizeof(T) == 8 ? (kViscousSO ? 1 : 2) : 1;
+
+template <bool SO, class Inv, class Visc, class Lim, class T>
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kStepsWaves<SO && Visc::isViscous, T>)))
+void k_steps(PersistArgs a, Inv inv, Visc visc) {
+  // TimeSolverExplicitRK4.h:187-195: alpha = {0, 1/2, 1/2, 1}, beta = {1/6,
+  // 1/3, 1/3, 1/6}, selected rather than indexed (an indexed array would sit
+  // in scratch)
+  auto beta = [](int k) { return k == 0 || k == 3 ? 1.0 / 6.0 : 1.0 / 3.0; };
+  auto alpha_next = [](int k) { return k == 2 ? 1.0 : k == 3 ? 0.0 : 1.0 / 2.0; };
+  constexpr bool FV = SO && !Visc::isViscous;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* lds = reinterpret_cast<T*>(lds_raw);  // flux face results, or the gradient records' staging
+  __shared__ double red[kTile / 64];
+  __shared__ int stop;
+  const DevMesh& m = a.m;
+  const int G = gridDim.x;
+  // XCD-aware walks as in k_flux: the workgroups of one XCD (every kXcds-th)
+  // take that XCD's contiguous run of tiles or blocks
+  const int xcd = blockIdx.x % kXcds, j0 = blockIdx.x / kXcds, xstride = G / kXcds;
+  const int nbf = m.nf - m.nif;
+  unsigned gen = 0;
+  for (int it = 0; it < a.nsteps; ++it) {
+    const int cur = a.cur0 ^ (it & 1);
+    double* Un = a.U[cur];
+    double* Uacc = a.U[1 - cur];
+    if (a.any_cfl) {
+      for (int tile = blockIdx.x; tile < a.t.ntiles; tile += G) {
+        dt_tile(m, a.t, tile, Un, a.viscous, a.prandtl, a.partial, red);
+        __syncthreads();
+      }
+      if (!grid_sync(a.gsync, gen, a.wait_ticks, &stop)) return;
+    }
+    for (int k = blockIdx.x; k < a.ncases; k += G) {
+      dt_case(a.partial, a.case_tiles, a.cfl, a.dt, a.time, k, red);
+      __syncthreads();
+    }
+    if (!grid_sync(a.gsync, gen, a.wait_ticks, &stop)) return;
+    if (it == 0 && a.prim0) {
+      for (int c = blockIdx.x * kTile + threadIdx.x; c < m.nc; c += G * kTile)
+        prim_cell(m.nc, Un, a.R, 0.0, static_cast<T*>(a.V[0]), c);
+      if (!grid_sync(a.gsync, gen, a.wait_ticks, &stop)) return;
+    }
+    for (int irk = 0; irk < 4; ++irk) {
+      const T* Vs = static_cast<const T*>(a.V[irk & 1]);
+      T* Vn = static_cast<T*>(a.V[1 - (irk & 1)]);
+      if (SO || Visc::isViscous) {
+        T* rec = FV ? nullptr : static_cast<T*>(a.grad);
+        for (int j = j0; j < a.per_blocks; j += xstride) {
+          const int b = xcd * a.per_blocks + j;
+          if (b * kBlock >= m.nc) break;  // uniform over the workgroup
+          const int c = b * kBlock + threadIdx.x;
+          T out[kRec];
+          if (c < m.nc) grad_cell<SO, FV, Lim, T>(m, a.nbr, Vs, rec, static_cast<T*>(a.fv), c, out);
+          if constexpr (!FV) {
+            store_records(rec, lds, out, b * kBlock, m.nc);
+            __syncthreads();  // the staging rows are reused by the next block
+          }
+        }
+        if (!grid_sync(a.gsync, gen, a.wait_ticks, &stop)) return;
+      }
+      if (nbf > 0) {
+        for (int i = blockIdx.x * kTile + threadIdx.x; i < nbf; i += G * kTile)
+          bc_face<Inv, ma::ReferenceBoundaries>(m, Vs, inv, visc.prandtl, a.inflow, a.bc_case,
+                                                static_cast<T*>(a.bc), i);
+        if (!grid_sync(a.gsync, gen, a.wait_ticks, &stop)) return;
+      }
+      const T* frec = static_cast<const T*>(FV ? a.fv : a.grad);
+      for (int j = j0; j < a.t.per_xcd; j += xstride) {
+        const int tile = xcd * a.t.per_xcd + j;
+        if (tile >= a.t.ntiles) break;  // uniform over the workgroup
+        flux_tile<SO, Inv, Visc, T>(m, a.t, tile, lds, Vs, frec, frec, static_cast<const T*>(a.bc),
+                                    irk == 0 ? Un : Uacc, Uacc, a.R, a.dt, beta(irk), inv, visc, Un, Vn,
+                                    alpha_next(irk), irk == 3 ? 1 : 0);
+        __syncthreads();  // the LDS is reused by the next tile
+      }
+      if (!grid_sync(a.gsync, gen, a.wait_ticks, &stop)) return;
+    }
+  }
+}
+
 // ------------------------------------------------------------- host side --
 
 #define MA_HIP(call)                                                                      \
@@ -1149,6 +1306,8 @@ struct ma_solver {
   IcParams* ic_dev = nullptr;
   Inflow* inflow_dev = nullptr;
   unsigned long long* counter = nullptr;
+  unsigned* gsync = nullptr;  // grid barrier of the persistent launch (k_steps)
+  bool persistent = false;    // ma_solver_step runs its steps as one k_steps launch
   double *norm_partial = nullptr, *norm_out = nullptr;  // per tile x 5, per case x 5
   const double* xc = nullptr;  // cell centroids (3 planes), for the initial condition
   const int* org = nullptr;    // device cell -> caller's cell index
@@ -2071,6 +2230,7 @@ ma_status build_planned(ma_solver* s, const RunPlan& P) {
        (st = s->alloc_bytes(&s->fv, (size_t)kMaxSlots * n5 * tsize))) ||
       (st = s->alloc_bytes(&s->bc, (size_t)5 * std::max<long long>(nbf, 1) * tsize)) ||
       (st = s->alloc(&s->partial, (size_t)std::max(1, s->t.ntiles))) || (st = s->alloc(&s->counter, 1)) ||
+      (st = s->alloc(&s->gsync, 4)) ||
       (st = s->alloc(&s->norm_partial, (size_t)5 * std::max(1, s->t.ntiles))) ||
       (st = s->alloc(&s->norm_out, (size_t)5 * ncases)))
     return st;
@@ -2200,6 +2360,100 @@ void dispatch(ma_solver* s, int n) {
     dispatch_t<double>(s, n);
 }
 
+// nsteps steps as one cooperative launch of k_steps: as many workgroups as
+// the device keeps resident at once (no more than the largest phase has
+// tiles or cell blocks), all of them walking every phase.  Waits for the
+// device to be idle before the launch and for the launch itself, since a
+// barrier wait that ran over its limit is reported here.
+template <bool SO, class Inv, class Visc, class Lim, class T>
+ma_status run_persistent(ma_solver* s, int nsteps) {
+  if (nsteps == 0) return MA_OK;
+  constexpr bool FV = SO && !Visc::isViscous;
+  const auto kern = k_steps<SO, Inv, Visc, Lim, T>;
+  const size_t lds = std::max((size_t)s->t.max_faces * 5 * sizeof(T),
+                              FV ? (size_t)0 : (size_t)kBlock * kStageRow<T> * sizeof(T));
+  int coop = 0, per_cu = 0, wall_khz = 0;
+  MA_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, s->device));
+  if (!coop) return ma::fail(MA_ERR_UNSUPPORTED, "persistent steps: the device has no cooperative launch");
+  MA_HIP(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, s->device));
+  hipFuncAttributes fa{};
+  MA_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(kern)));
+  if (fa.localSizeBytes > 0)
+    return ma::fail(MA_ERR_UNSUPPORTED, "persistent steps: the kernel uses scratch memory, which can keep "
+                                        "workgroups of a cooperative launch from being resident together");
+  MA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kTile, lds));
+  const int nc = s->m.nc;
+  const int most = std::max(std::max(s->t.ntiles, nblocks(nc)), std::max(nblocks(s->m.nf - s->m.nif), s->ncases));
+  int grid = std::min(per_cu * s->cus, (most + kXcds - 1) / kXcds * kXcds);
+  grid -= grid % kXcds;
+  if (grid < kXcds || wall_khz <= 0)
+    return ma::fail(MA_ERR_UNSUPPORTED, "persistent steps: too few resident workgroups");
+  PersistArgs a{};
+  a.m = s->m;
+  a.t = s->t;
+  a.nbr = s->nbr;
+  a.V[0] = s->V;
+  a.V[1] = s->V2;
+  a.grad = s->grad;
+  a.fv = s->fv;
+  a.bc = s->bc;
+  a.U[0] = s->U[0];
+  a.U[1] = s->U[1];
+  a.R = s->R;
+  a.dt = s->dt_dev;
+  a.time = s->time_dev;
+  a.cfl = s->cfl_dev;
+  a.partial = s->partial;
+  a.case_tiles = s->case_tiles;
+  a.inflow = s->inflow_dev;
+  a.bc_case = s->bc_case;
+  a.gsync = s->gsync;
+  a.wait_ticks = (long long)wall_khz * 1000 * 10;  // 10 s
+  a.ncases = s->ncases;
+  a.nsteps = nsteps;
+  a.cur0 = s->cur;
+  a.prim0 = s->v0_valid ? 0 : 1;
+  a.any_cfl = s->any_cfl ? 1 : 0;
+  a.viscous = s->opt.viscous;
+  a.per_blocks = (nblocks(nc) + kXcds - 1) / kXcds;
+  a.prandtl = effective_prandtl(s->opt);
+  Visc visc{a.prandtl};
+  Inv inv{};
+  void* args[] = {&a, &inv, &visc};
+  // The grid is sized to every workgroup slot of the device, so no other
+  // kernel of the process may hold slots while it is placed: wait for the
+  // whole device (other solvers' streams included) first.
+  MA_HIP(hipDeviceSynchronize());
+  MA_HIP(hipMemsetAsync(s->gsync, 0, 4 * sizeof(unsigned), s->stream));
+  MA_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3(grid), dim3(kTile), args, lds,
+                                    s->stream));
+  unsigned exceeded = 0;
+  MA_HIP(hipMemcpyAsync(&exceeded, s->gsync + 2, sizeof(exceeded), hipMemcpyDeviceToHost, s->stream));
+  MA_HIP(hipStreamSynchronize(s->stream));
+  if (exceeded) return ma::fail(MA_ERR_DEVICE, "persistent steps: a grid barrier wait ran over its limit");
+  s->cur ^= nsteps & 1;  // U_acc after the last stage is the new sol_n, every step
+  s->steps_done += nsteps;
+  s->v0_valid = true;
+  return MA_OK;
+}
+
+template <bool SO, class Visc, class T>
+ma_status persist_lim(ma_solver* s, int n) {
+  if (s->opt.limiter == MA_LIMITER_VANALBADA) return run_persistent<SO, Inviscid, Visc, ma::VanAlbadaLimiter, T>(s, n);
+  return run_persistent<SO, Inviscid, Visc, ma::VenkatLimiter, T>(s, n);
+}
+
+template <class T>
+ma_status persist_t(ma_solver* s, int n) {
+  const bool so = s->opt.second_order_space != 0, visc = s->opt.viscous != 0;
+  if (so && visc) return persist_lim<true, ma::NewtonianViscousFlux, T>(s, n);
+  if (so) return persist_lim<true, ma::NoViscousFlux, T>(s, n);
+  if (visc) return persist_lim<false, ma::NewtonianViscousFlux, T>(s, n);
+  return persist_lim<false, ma::NoViscousFlux, T>(s, n);
+}
+
+ma_status persist(ma_solver* s, int n) { return s->fp32 ? persist_t<float>(s, n) : persist_t<double>(s, n); }
+
 void drop_graphs(ma_solver* s) {
   for (auto& g : s->graph)
     if (g) {
@@ -2253,6 +2507,7 @@ ma_status advance(ma_solver* s, int nsteps) {
                     "graph steps are captured and replayed only on the thread that created the solver; "
                     "call ma_solver_set_graph(s, 0) to step from another thread");
   if (nsteps > 0 && s->opt.second_order_space && !s->opt.viscous) s->grads_stale = true;
+  if (s->persistent) return persist(s, nsteps);
   if (s->graph_steps > 0) {
     while (nsteps >= s->graph_steps) {
       ma_status st0 = ensure_stage0_prims(s);
@@ -2559,6 +2814,19 @@ extern "C" ma_status ma_solver_set_graph(ma_solver* s, int steps_per_graph) {
   MA_HIP(hipStreamSynchronize(s->stream));
   drop_graphs(s);
   s->graph_steps = steps_per_graph;
+  if (steps_per_graph > 0) s->persistent = false;
+  return MA_OK;
+}
+
+extern "C" ma_status ma_solver_set_persistent(ma_solver* s, int enabled) {
+  if (!s || enabled < 0 || enabled > 1)
+    return ma::fail(MA_ERR_INVALID, "ma_solver_set_persistent: enabled must be 0 or 1");
+  MA_HIP(hipStreamSynchronize(s->stream));
+  if (enabled) {
+    drop_graphs(s);
+    s->graph_steps = 0;
+  }
+  s->persistent = enabled != 0;
   return MA_OK;
 }
 

@@ -210,6 +210,7 @@ class BatchSolver {
   void initialize() { check(ma_solver_initialize(s_)); }
   void step(int n) { check(ma_solver_step(s_, n)); }
   void set_graph(int steps_per_graph) { check(ma_solver_set_graph(s_, steps_per_graph)); }
+  void set_persistent(bool enabled) { check(ma_solver_set_persistent(s_, enabled ? 1 : 0)); }
   int cases() const { return (int)cells_.size(); }
   std::vector<double> case_state(int k) {
     std::vector<double> U((size_t)cells_.at(k) * 5);

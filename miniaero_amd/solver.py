@@ -84,6 +84,11 @@ class TimeSolverExplicitRK4:
         """Run groups of steps_per_graph steps (even) as one captured HIP graph; 0 = off."""
         _lib.check(self._L.ma_solver_set_graph(self._h, int(steps_per_graph)))
 
+    def set_persistent(self, enabled: bool) -> None:
+        """Run each step() call's steps as one cooperative launch with grid-wide
+        barriers between phases (waits for it); turns graphs off."""
+        _lib.check(self._L.ma_solver_set_persistent(self._h, int(bool(enabled))))
+
     def synchronize(self) -> None:
         _lib.check(self._L.ma_solver_synchronize(self._h))
 

@@ -153,6 +153,58 @@ def test_graph_replays_match_plain_launches(graph_steps, scheme):
         b.close()
 
 
+@pytest.mark.parametrize("scheme,precision", [("viscous", 0), ("viscous", 1), ("inviscid", 0), ("inviscid", 1),
+                                              ("first_order", 0), ("first_order_viscous", 1)])
+def test_persistent_steps_match_plain_launches(scheme, precision):
+    """Steps run as one cooperative launch with grid-wide barriers between the
+    phases (ma_solver_set_persistent) equal plain launches bit for bit: state,
+    time and last dt of every case, over odd and even step counts, after
+    set_state (the launch then computes the first stage's primitives itself)
+    and after the residual and gradients are read between calls.  The
+    inviscid batch holds a 524k-cell case, more flux tiles than the launch
+    has resident workgroups, so each workgroup walks several."""
+    big = dict(VANALBADA_CASES[0], nx=128, ny=64, nz=64, dt=2e-5)
+    kws = {"viscous": [VISCOUS_CASES[1], VISCOUS_CASES[2], VISCOUS_CASES[4]],
+           "inviscid": VANALBADA_CASES[:3] + [big],
+           "first_order": [dict(k, second_order_space=0) for k in VANALBADA_CASES[:2] + [big]],
+           "first_order_viscous": [dict(k, second_order_space=0) for k in VISCOUS_CASES[1:3]]}[scheme]
+    kws = [dict(k, precision=precision) for k in kws]
+    cases = [make_case(kw) for kw in kws]
+    a = M.BatchSolver(cases)
+    b = M.BatchSolver(cases)
+    try:
+        a.initialize()
+        b.initialize()
+        b.set_persistent(True)
+        for n in (3, 1, 4, 2):
+            a.step(n)
+            b.step(n)
+            Ua, Ub = a.state(), b.state()
+            print(f"[gpu-batch] persistent {scheme} fp{64 - 32 * precision}: +{n} steps, "
+                  f"bitwise={np.array_equal(Ua, Ub)}", flush=True)
+            assert np.array_equal(Ua, Ub)
+            for k in range(len(cases)):
+                assert a.case_time(k) == b.case_time(k)
+        assert np.array_equal(a.residual(), b.residual())
+        ga, gb = a.gradients(), b.gradients()
+        assert all(np.array_equal(x, y) for x, y in zip(ga, gb))
+        U = a.state()
+        U[:, 0] *= 1.0 + 1e-3 * np.cos(np.arange(len(U)))
+        a.set_state(U)
+        b.set_state(U)
+        a.step(3)
+        b.step(3)
+        assert np.array_equal(a.state(), b.state())
+        assert b.nonfinite() == 0
+        b.set_persistent(False)  # back to plain launches
+        a.step(1)
+        b.step(1)
+        assert np.array_equal(a.state(), b.state())
+    finally:
+        a.close()
+        b.close()
+
+
 def test_graph_steps_only_on_the_creating_thread():
     """Graph capture and replay are confined to the thread that created the
     solver: from another thread a step that would use a graph is refused
