@@ -541,8 +541,9 @@ def _compact(r, steps) -> dict:
 
 def sod_c1_full(M) -> dict:
     """The 3D Sod case (64x16x16, first order, 100 steps) run in full on the
-    GPU and in full on one host core by the CPU oracle; the two final states
-    are compared bit for bit."""
+    GPU -- with a launch per kernel, as one captured graph of all 100 steps
+    and as one persistent launch -- and in full on one host core by the CPU
+    oracle; every final state is compared with the oracle's bit for bit."""
     import dataclasses
 
     import numpy as np
@@ -552,17 +553,26 @@ def sod_c1_full(M) -> dict:
 
     o = sod_c1()
     md = M.Parallel3DMesh(o.nx, o.ny, o.nz, o.lx, o.ly, o.lz, o.problem_type, o.angle, options=o).fillMeshData()
-    s = M.TimeSolverExplicitRK4(md, o)
-    s.initialize()
-    s.step(2)
-    s.synchronize()
-    s.initialize()
-    t0 = time.perf_counter()
-    s.step(o.ntimesteps)
-    s.synchronize()
-    gpu_s = time.perf_counter() - t0
-    U = s.state()
-    s.close()
+
+    def run(mode):
+        # the whole case, after an untimed run of it (which captures the graph)
+        s = M.TimeSolverExplicitRK4(md, o)
+        try:
+            if mode == "graph":
+                s.set_graph(o.ntimesteps)
+            elif mode == "persistent":
+                s.set_persistent(True)
+            s.initialize()
+            s.step(o.ntimesteps)
+            s.synchronize()
+            s.initialize()
+            t0 = time.perf_counter()
+            s.step(o.ntimesteps)
+            s.synchronize()
+            return time.perf_counter() - t0, s.state()
+        finally:
+            s.close()
+
     kw = {f.name: getattr(o, f.name) for f in dataclasses.fields(o)
           if f.name not in ("output_results", "output_frequency")}
     om = O.OracleMesh(O.make_options(**kw))
@@ -571,11 +581,20 @@ def sod_c1_full(M) -> dict:
     t0 = time.perf_counter()
     os_.step(o.ntimesteps)
     cpu_s = time.perf_counter() - t0
+    Uref = os_.state()
     work = om.ncells * 4 * o.ntimesteps
+    modes = {}
+    for mode in ("per_kernel", "graph", "persistent"):
+        sec, U = run(mode)
+        modes[mode] = {"value": work / sec, "ms_per_step": sec / o.ntimesteps * 1e3,
+                       "matches_oracle_bitwise": bool(np.array_equal(U, Uref))}
+    best = max(modes, key=lambda m: modes[m]["value"])
+    gpu_s = work / modes[best]["value"]
     return {"value": work / gpu_s, "unit": "cell-stage updates/s", "ms_per_step": gpu_s / o.ntimesteps * 1e3,
             "steps": o.ntimesteps, "dtype": "f64",
-            "note": "16k cells: the GPU time is launch latency, not bandwidth",
-            "matches_oracle_bitwise": bool(np.array_equal(U, os_.state())),
+            "note": "16k cells: the GPU time is launch latency, not bandwidth; value = the fastest step mode",
+            "step_mode": best, "step_modes": modes,
+            "matches_oracle_bitwise": all(m["matches_oracle_bitwise"] for m in modes.values()),
             "cpu_baseline": {"value": work / cpu_s, "unit": "cell-stage updates/s", "cores": 1, "kind": "port",
                              "sample": f"the whole case ({om.ncells} cells, {o.ntimesteps} RK4 steps) on one "
                                        f"core by the oracle (single-thread C++ restatement of the reference's "

@@ -31,11 +31,15 @@ def _mesh(opts):
                             opts.angle, options=opts).fillMeshData()
 
 
-def _against_oracle(opts, chunks, per_chunk):
+def _against_oracle(opts, chunks, per_chunk, mode="per_kernel"):
     md = _mesh(opts)
     s = M.TimeSolverExplicitRK4(md, opts)
     cpu = O.OracleSolver(O.OracleMesh(O.make_options(**_kw(opts))))
     try:
+        if mode == "graph":
+            s.set_graph(per_chunk)
+        elif mode == "persistent":
+            s.set_persistent(True)
         s.initialize()
         cpu.init()
         assert np.array_equal(s.state(), cpu.state()), "initial state"
@@ -51,9 +55,12 @@ def _against_oracle(opts, chunks, per_chunk):
         s.close()
 
 
-def test_c1_sod_in_full_matches_oracle():
+@pytest.mark.parametrize("mode", ["per_kernel", "graph", "persistent"])
+def test_c1_sod_in_full_matches_oracle(mode):
+    """C1 in full, checked every 10 steps, with a launch per kernel, as graph
+    replays of 10 steps and as one persistent launch per 10 steps."""
     o = sod_c1()
-    _against_oracle(o, chunks=10, per_chunk=o.ntimesteps // 10)
+    _against_oracle(o, chunks=10, per_chunk=o.ntimesteps // 10, mode=mode)
 
 
 @pytest.mark.slow
