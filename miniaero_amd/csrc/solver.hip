@@ -61,6 +61,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -71,6 +72,7 @@
 #include <new>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/miniaero.h"
@@ -112,6 +114,11 @@ struct DevMesh {
   // CFL runs: 5 planes of nc, each cell's face geometry S_x, S_y, S_z, N1, N2
   // (see dt_tile); null otherwise
   const double* __restrict__ dtg;
+  // Geometry records of the internal faces in the arithmetic type,
+  // kGeoStride values each: area vector, T(x_f - x_left), T(x_f - x_right),
+  // the same values as the fn/fn32 and fdx planes; faces with the same
+  // values share a record (TileMap::fgid)
+  const void* __restrict__ gtab;
 };
 
 struct Inflow {
@@ -131,6 +138,7 @@ struct TileMap {
   const int* __restrict__ slot;   // kMaxSlots planes of nc, see k_flux
   const int* __restrict__ cptr;   // ntiles + 1: cells of tile b are [cptr[b], cptr[b+1])
   const int* __restrict__ tcase;  // ntiles: the case (independent mesh) tile b belongs to
+  const int* __restrict__ fgid;   // parallel to faces: the face's record in DevMesh::gtab
 };
 
 template <class S, class T>
@@ -207,6 +215,50 @@ __device__ __forceinline__ void store_records(T* __restrict__ rec, T* stage, con
       *reinterpret_cast<double2*>(dst) = make_double2(src[0], src[1]);
     else
       *reinterpret_cast<float4*>(dst) = make_float4(src[0], src[1], src[2], src[3]);
+  }
+}
+
+// Shared face-geometry record: area vector, x_f - x_left, x_f - x_right
+// (kGeoVals values), padded to whole 16-byte chunks (kGeoStride values: 10
+// in FP64, 12 in FP32) so that one record is a few vector loads.
+constexpr int kGeoVals = 9;
+template <class T>
+constexpr int kGeoStride = (kGeoVals * (int)sizeof(T) + 15) / 16 * 16 / (int)sizeof(T);
+
+// Record g of the shared geometry table: the area vector, and with kDisp
+// both displacements.
+template <class T, bool kDisp>
+__device__ __forceinline__ void load_geo(const DevMesh& m, int g, T* a, T* dl, T* dr) {
+  if constexpr (sizeof(T) == 8) {
+    const double2* p = reinterpret_cast<const double2*>(m.gtab) + (size_t)g * (kGeoStride<T> / 2);
+    const double2 x0 = p[0], x1 = p[1];
+    a[0] = x0.x;
+    a[1] = x0.y;
+    a[2] = x1.x;
+    if constexpr (kDisp) {
+      const double2 x2 = p[2], x3 = p[3], x4 = p[4];
+      dl[0] = x1.y;
+      dl[1] = x2.x;
+      dl[2] = x2.y;
+      dr[0] = x3.x;
+      dr[1] = x3.y;
+      dr[2] = x4.x;
+    }
+  } else {
+    const float4* p = reinterpret_cast<const float4*>(m.gtab) + (size_t)g * (kGeoStride<T> / 4);
+    const float4 x0 = p[0];
+    a[0] = x0.x;
+    a[1] = x0.y;
+    a[2] = x0.z;
+    if constexpr (kDisp) {
+      const float4 x1 = p[1], x2 = p[2];
+      dl[0] = x0.w;
+      dl[1] = x1.x;
+      dl[2] = x1.y;
+      dr[0] = x1.z;
+      dr[1] = x1.w;
+      dr[2] = x2.x;
+    }
   }
 }
 
@@ -565,12 +617,12 @@ void k_grad(DevMesh m, const int* __restrict__ nbr, const T* __restrict__ V, T* 
 template <bool kSecondOrder, class Inv, class Visc, class T>
 __device__ __forceinline__ void interior_flux(const DevMesh& m, const T* __restrict__ V,
                                               const T* __restrict__ rec, const T* __restrict__ fv, int f,
-                                              int L, int sL, int Rr, int sR, const Inv& inv, const Visc& visc,
-                                              T* F) {
+                                              int g, int L, int sL, int Rr, int sR, const Inv& inv,
+                                              const Visc& visc, T* F) {
   constexpr bool kFaceVals = kSecondOrder && !Visc::isViscous;
   constexpr bool kRecon = kSecondOrder && !kFaceVals;  // extrapolated here
   constexpr bool kGrad = Visc::isViscous;
-  const int nc = m.nc, nf = m.nf;
+  const int nc = m.nc;
   T vl[5], vr[5], a[3];
   T gl[kGrad ? 15 : 1], gr[kGrad ? 15 : 1], pl[5], pr[5], dl[3], dr[3];
   if (kFaceVals) {
@@ -580,7 +632,7 @@ __device__ __forceinline__ void interior_flux(const DevMesh& m, const T* __restr
     load5(V, nc, L, vl);
     load5(V, nc, Rr, vr);
   }
-  load3(area<T>(m), nf, f, a);
+  load_geo<T, kRecon>(m, g, a, dl, dr);  // the face's geometry record
   if (kGrad) {  // (kRecon implies kGrad)
     T ra[kRec], rb[kRec];
     load_rec(rec, L, ra);
@@ -594,14 +646,6 @@ __device__ __forceinline__ void interior_flux(const DevMesh& m, const T* __restr
     for (int k = 0; k < 5; ++k) {
       pl[k] = ra[15 + k];
       pr[k] = rb[15 + k];
-    }
-  }
-  if (kRecon) {
-    const T* dx = static_cast<const T*>(m.fdx);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      dl[d] = dx[(size_t)d * nf + f];
-      dr[d] = dx[(size_t)(3 + d) * nf + f];
     }
   }
   const ma::FaceFrameT<T> fr = frame_of<T>(m, f, a);
@@ -710,21 +754,24 @@ __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, in
   // the next face's indices are fetched while the current face is evaluated
   int f = 0;
   int2 lr = make_int2(0, 0);
+  int g = 0;
   if (threadIdx.x < nfl) {
     f = t.faces[f0 + threadIdx.x];
     lr = t.lr[f0 + threadIdx.x];
+    g = t.fgid[f0 + threadIdx.x];
   }
   for (int i = threadIdx.x; i < nfl; i += kTile) {
-    const int fc = f;
+    const int fc = f, gc = g;
     const int2 lrc = lr;
     if (i + kTile < nfl) {
       f = t.faces[f0 + i + kTile];
       lr = t.lr[f0 + i + kTile];
+      g = t.fgid[f0 + i + kTile];
     }
     T F[5];
     // slots are packed into the cell entries only for the face-value scheme
     constexpr bool kFaceVals = kSecondOrder && !Visc::isViscous;
-    interior_flux<kSecondOrder>(m, V, rec, fv, fc, kFaceVals ? cell_of(lrc.x) : lrc.x, slot_of(lrc.x),
+    interior_flux<kSecondOrder>(m, V, rec, fv, fc, gc, kFaceVals ? cell_of(lrc.x) : lrc.x, slot_of(lrc.x),
                                 kFaceVals ? cell_of(lrc.y) : lrc.y, slot_of(lrc.y), inv, visc, F);
 #pragma unroll
     for (int k = 0; k < 5; ++k) lds[k * mf + i] = F[k];
@@ -801,9 +848,11 @@ __device__ __forceinline__ void flux_tile(const DevMesh& m, const TileMap& t, in
 
 // Waves per SIMD the flux kernels are register-budgeted for: FP64 3 (the
 // compiler's own choice lands one register over the 168 that allow 3, at 2
-// waves, measured 10% slower); FP32 no constraint.
+// waves, measured 10% slower); FP32 5 (the viscous multi-case kernel would
+// otherwise take 98 registers, 4 waves; the others use fewer than 5 waves'
+// budget anyway).
 template <class T>
-constexpr int kFluxWaves = sizeof(T) == 8 ? 3 : 1;
+constexpr int kFluxWaves = sizeof(T) == 8 ? 3 : 5;
 
 template <bool kSecondOrder, class Inv, class Visc, class T>
 __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(kFluxWaves<T>)))
@@ -1554,6 +1603,10 @@ struct Layout {
   std::vector<int> dev_of, org_of, tile_cptr, tptr, tfaces, slot, tslot, fl, fr;
   std::vector<double> fn, ffr, fx, fwall, vol, xc;
   std::vector<signed char> fkind;
+  // geometry records of the internal faces (see dedupe_geometry): the record
+  // of each internal face, and kGeoVals FP64 values per record
+  std::vector<int> fgeo;
+  std::vector<double> geo;
 };
 
 // Every index a kernel will use, checked on the host before any upload.
@@ -1583,6 +1636,13 @@ ma_status validate_layout(const Layout& L) {
     hit[dev_of[c]] = 1;
   }
   if (tile_cptr.front() != 0 || tile_cptr.back() != nc) return ma::fail(MA_ERR_INVALID, "layout: tile cells");
+  {
+    const size_t ng = L.geo.size() / kGeoVals;
+    if ((int)L.fgeo.size() != nif || L.geo.size() % kGeoVals || (nif > 0 && ng == 0))
+      return ma::fail(MA_ERR_INVALID, "layout: geometry records");
+    for (int x : L.fgeo)
+      if (x < 0 || (size_t)x >= ng) return ma::fail(MA_ERR_INVALID, "layout: geometry record index");
+  }
   for (int b = 0; b < ntiles; ++b) {
     const int n = tile_cptr[b + 1] - tile_cptr[b];
     if (n < 1 || n > kTile) return ma::fail(MA_ERR_INVALID, "layout: tile size");
@@ -1651,6 +1711,8 @@ void merge_layouts(std::vector<Layout>& parts, Layout& M, std::vector<int>& tile
   M.fx.assign((size_t)3 * nf, 0.0);
   M.vol.assign(nc, 0.0);
   M.xc.assign((size_t)3 * nc, 0.0);
+  M.fgeo.assign(M.nif, 0);
+  M.geo.clear();
   tile_case.assign(M.ntiles, 0);
   case_tiles.assign(n + 1, 0);
   bc_case.assign(std::max(1, M.nbf), 0);
@@ -1705,7 +1767,70 @@ void merge_layouts(std::vector<Layout>& parts, Layout& M, std::vector<int>& tile
       }
       if (f >= lnif) bc_case[bo[k] + (f - lnif)] = k;
     }
+    {  // this part's geometry records follow the earlier parts'
+      const int r0 = (int)(M.geo.size() / kGeoVals);
+      for (int f = 0; f < lnif; ++f) M.fgeo[io[k] + f] = r0 + L.fgeo[f];
+      M.geo.insert(M.geo.end(), L.geo.begin(), L.geo.end());
+    }
     Layout().swap_out(L);  // release the part as soon as it is merged
+  }
+}
+
+// Geometry records of the internal faces, which the flux kernel reads: each
+// face's area vector and both displacements x_f - x_left, x_f - x_right
+// (FP64, the expressions the displacement planes use), with faces whose nine
+// values have the same bits sharing one record.  Box meshes repeat few
+// records over many faces (the 256 x 128 x 64 stretched plate: 7,534 records
+// for 6.2M internal faces; a uniform box: 3), so the kernel reads a 4-byte
+// record index per face instead of 9 values, and the records stay in L2.
+// When records repeat fewer than kGeoReuse times on average, every face gets
+// its own record (record f for face f).
+constexpr int kGeoReuse = 8;
+
+struct GeoKeyHash {
+  size_t operator()(const std::array<unsigned long long, kGeoVals>& k) const {
+    unsigned long long h = 1469598103934665603ull;
+    for (unsigned long long x : k) {
+      h ^= x;
+      h *= 1099511628211ull;
+      h ^= h >> 29;
+    }
+    return (size_t)h;
+  }
+};
+
+void dedupe_geometry(Layout& L) {
+  const int nif = L.nif, nf = L.nf, nc = L.nc;
+  auto record = [&](int f, double* r) {
+    for (int d = 0; d < 3; ++d) {
+      r[d] = L.fn[(size_t)d * nf + f];
+      r[3 + d] = L.fx[(size_t)d * nf + f] - L.xc[(size_t)d * nc + L.fl[f]];
+      r[6 + d] = L.fx[(size_t)d * nf + f] - L.xc[(size_t)d * nc + L.fr[f]];
+    }
+  };
+  L.fgeo.assign((size_t)nif, 0);
+  L.geo.clear();
+  const size_t cap = std::min<size_t>((size_t)nif / kGeoReuse, size_t(1) << 20);
+  std::unordered_map<std::array<unsigned long long, kGeoVals>, int, GeoKeyHash> ids;
+  ids.reserve(std::min<size_t>(cap, 1 << 16) * 2);
+  bool shared = nif >= kGeoReuse;
+  for (int f = 0; shared && f < nif; ++f) {
+    double r[kGeoVals];
+    record(f, r);
+    std::array<unsigned long long, kGeoVals> key;
+    std::memcpy(key.data(), r, sizeof r);
+    const auto it = ids.emplace(key, (int)ids.size());
+    if (it.second) {
+      if (ids.size() > cap) shared = false;  // too few repeats
+      L.geo.insert(L.geo.end(), r, r + kGeoVals);
+    }
+    L.fgeo[f] = it.first->second;
+  }
+  if (shared) return;
+  L.geo.assign((size_t)kGeoVals * nif, 0.0);
+  for (int f = 0; f < nif; ++f) {
+    record(f, &L.geo[(size_t)kGeoVals * f]);
+    L.fgeo[f] = f;
   }
 }
 
@@ -1910,6 +2035,7 @@ ma_status plan_layout(const ma_mesh_data* md, const TileShape& shape, Layout& L)
   }
   L.ntiles = ntiles;
   L.max_faces = max_faces;
+  dedupe_geometry(L);
   return MA_OK;
 }
 
@@ -1963,6 +2089,26 @@ ma_status upload_disp(ma_solver* s, const Layout& L, void** out) {
   }
   T* p = nullptr;
   ma_status st = s->counting ? s->alloc(&p, (size_t)6 * nf) : upload(s, &p, h);
+  *out = p;
+  return st;
+}
+
+// Uploads the shared geometry records in the arithmetic type, kGeoStride
+// values each (T of the FP64 values, as the fn32 and displacement planes
+// round them).
+template <class T>
+ma_status upload_geo(ma_solver* s, const Layout& L, void** out) {
+  const size_t ng = L.geo.size() / kGeoVals;
+  std::vector<T> h;
+  if (!s->counting) {
+    h.assign(ng * kGeoStride<T>, T(0));
+    for (size_t r = 0; r < ng; ++r)
+      for (int v = 0; v < kGeoVals; ++v) h[r * kGeoStride<T> + v] = T(L.geo[r * kGeoVals + v]);
+  } else {
+    h.resize(1);
+  }
+  T* p = nullptr;
+  ma_status st = s->counting ? s->alloc(&p, ng * kGeoStride<T>) : upload(s, &p, h);
   *out = p;
   return st;
 }
@@ -2156,6 +2302,23 @@ ma_status build_planned(ma_solver* s, const RunPlan& P) {
   s->t.slot = dtslot;
   s->t.cptr = dtcptr;
   s->t.tcase = dtcase;
+  // face geometry records, and each tile face's record index
+  {
+    void* dg = nullptr;
+    if ((st = s->fp32 ? upload_geo<float>(s, L, &dg) : upload_geo<double>(s, L, &dg))) return st;
+    int* dfg = nullptr;
+    const size_t ntf = L.tfaces.size();
+    if (s->counting) {
+      st = s->alloc(&dfg, ntf);
+    } else {
+      std::vector<int> h(ntf, 0);
+      for (size_t i = 0; i < (size_t)L.tptr[L.ntiles]; ++i) h[i] = L.fgeo[L.tfaces[i]];
+      st = upload(s, &dfg, h);
+    }
+    if (st) return st;
+    m.gtab = dg;
+    s->t.fgid = dfg;
+  }
 
   // per-case parameters
   std::vector<double> dt0(ncases), cfl(ncases), zero(ncases, 0.0);
@@ -3163,7 +3326,8 @@ ma_status plan_layout_shared(const ma_mesh_data* md, const ma_options* opt, std:
     lp->host_bytes = vbytes(L.dev_of) + vbytes(L.org_of) + vbytes(L.tile_cptr) + vbytes(L.tptr) +
                      vbytes(L.tfaces) + vbytes(L.slot) + vbytes(L.tslot) + vbytes(L.fl) + vbytes(L.fr) +
                      vbytes(L.fn) + vbytes(L.ffr) + vbytes(L.fx) + vbytes(L.fwall) + vbytes(L.vol) +
-                     vbytes(L.xc) + vbytes(L.fkind) + vbytes(lp->P.tile_case) + vbytes(lp->P.case_tiles) +
+                     vbytes(L.xc) + vbytes(L.fkind) + vbytes(L.fgeo) + vbytes(L.geo) +
+                     vbytes(lp->P.tile_case) + vbytes(lp->P.case_tiles) +
                      vbytes(lp->P.bc_case) + vbytes(lp->P.cell_case) + vbytes(lp->P.case_cell_off);
     *cells = L.nc;
     *host_bytes = lp->host_bytes;
