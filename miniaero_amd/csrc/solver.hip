@@ -114,11 +114,14 @@ struct DevMesh {
   // CFL runs: 5 planes of nc, each cell's face geometry S_x, S_y, S_z, N1, N2
   // (see dt_tile); null otherwise
   const double* __restrict__ dtg;
-  // Geometry records of the internal faces in the arithmetic type,
-  // kGeoStride values each: area vector, T(x_f - x_left), T(x_f - x_right),
-  // the same values as the fn/fn32 and fdx planes; faces with the same
-  // values share a record (TileMap::fgid)
+  // Geometry records of the faces in the arithmetic type, kGeoStride values
+  // each: area vector, T(x_f - x_left), T(x_f - x_right) (zero for boundary
+  // faces), the same values as the fn/fn32 and fdx planes; faces with the
+  // same values share a record (TileMap::fgid, sgeo)
   const void* __restrict__ gtab;
+  // kMaxSlots planes of nc: (record << 1) | side of each slot's face, -1 for
+  // an empty slot (the gradient kernel's slot list)
+  const int* __restrict__ sgeo;
 };
 
 struct Inflow {
@@ -229,13 +232,19 @@ constexpr int kGeoStride = (kGeoVals * (int)sizeof(T) + 15) / 16 * 16 / (int)siz
 // both displacements.
 template <class T, bool kDisp>
 __device__ __forceinline__ void load_geo(const DevMesh& m, int g, T* a, T* dl, T* dr) {
-  if constexpr (sizeof(T) == 8) {
+  if constexpr (sizeof(T) == 8 && !kDisp) {
+    const double2* p = reinterpret_cast<const double2*>(m.gtab) + (size_t)g * (kGeoStride<T> / 2);
+    const double2 x0 = p[0];
+    a[0] = x0.x;
+    a[1] = x0.y;
+    a[2] = reinterpret_cast<const double*>(p)[2];
+  } else if constexpr (sizeof(T) == 8) {
     const double2* p = reinterpret_cast<const double2*>(m.gtab) + (size_t)g * (kGeoStride<T> / 2);
     const double2 x0 = p[0], x1 = p[1];
     a[0] = x0.x;
     a[1] = x0.y;
     a[2] = x1.x;
-    if constexpr (kDisp) {
+    {
       const double2 x2 = p[2], x3 = p[3], x4 = p[4];
       dl[0] = x1.y;
       dl[1] = x2.x;
@@ -244,13 +253,17 @@ __device__ __forceinline__ void load_geo(const DevMesh& m, int g, T* a, T* dl, T
       dr[1] = x3.y;
       dr[2] = x4.x;
     }
+  } else if constexpr (!kDisp) {  // three values: no register for a fourth
+    const float* p = static_cast<const float*>(m.gtab) + (size_t)g * kGeoStride<T>;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) a[k] = p[k];
   } else {
     const float4* p = reinterpret_cast<const float4*>(m.gtab) + (size_t)g * (kGeoStride<T> / 4);
     const float4 x0 = p[0];
     a[0] = x0.x;
     a[1] = x0.y;
     a[2] = x0.z;
-    if constexpr (kDisp) {
+    {
       const float4 x1 = p[1], x2 = p[2];
       dl[0] = x0.w;
       dl[1] = x1.x;
@@ -260,6 +273,14 @@ __device__ __forceinline__ void load_geo(const DevMesh& m, int g, T* a, T* dl, T
       dr[2] = x2.x;
     }
   }
+}
+
+// Displacement x_f - x_c of side `side` of record g.
+template <class T>
+__device__ __forceinline__ void load_geo_disp(const DevMesh& m, int g, int side, T* d) {
+  const T* p = static_cast<const T*>(m.gtab) + (size_t)g * kGeoStride<T> + 3 + 3 * side;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) d[k] = p[k];
 }
 
 // Area vectors in the arithmetic type.
@@ -402,14 +423,6 @@ __global__ __launch_bounds__(kBlock) void k_prim(int nc, const double* __restric
   if (c < nc) prim_cell(nc, Un, R, alpha, V, c);
 }
 
-// Displacement x_f - x_c of the face in slot entry `ent` (side ent & 1), from
-// the FP64 centroids rounded to T; an empty slot reads face 0.
-template <class T>
-__device__ __forceinline__ void load_disp(const DevMesh& m, int ent, T* d) {
-  const int e = ent < 0 ? 0 : ent;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) d[k] = static_cast<const T*>(m.fdx)[(size_t)((e & 1) * 3 + k) * m.nf + (e >> 1)];
-}
 
 // Green-Gauss gradient (GreenGauss.h:51-270), then, for second order, the
 // stencil min/max (StencilLimiter.h:56-282) and the limiter
@@ -437,11 +450,12 @@ __device__ __forceinline__ void load_disp(const DevMesh& m, int ent, T* d) {
 template <bool kSecondOrder, bool kFaceVals, class Limiter, class T>
 __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restrict__ nbr, const T* __restrict__ V,
                                           T* __restrict__ rec, T* __restrict__ fv, int c, T* out) {
-  const int nc = m.nc, nf = m.nf;
+  const int nc = m.nc;
+  // slot entries: (geometry record << 1) | side, -1 empty
   int ent[kMaxSlots], nb[kMaxSlots];
 #pragma unroll
   for (int s = 0; s < kMaxSlots; ++s) {
-    ent[s] = m.slot[(size_t)s * nc + c];
+    ent[s] = m.sgeo[(size_t)s * nc + c];
     nb[s] = nbr[(size_t)s * nc + c];
   }
   T vc[5];
@@ -461,10 +475,10 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
     mx[k] = -T(1.0e300);
   }
 
-  T an[kMaxSlots][3], vnn[kMaxSlots][5];  // empty slots read face 0 and the cell itself
+  T an[kMaxSlots][3], vnn[kMaxSlots][5];  // empty slots read record 0 and the cell itself
 #pragma unroll
   for (int s = 0; s < kMaxSlots; ++s) {
-    load3(area<T>(m), nf, ent[s] < 0 ? 0 : ent[s] >> 1, an[s]);
+    load_geo<T, false>(m, ent[s] < 0 ? 0 : ent[s] >> 1, an[s], nullptr, nullptr);
     load5(V, nc, nb[s], vnn[s]);
   }
   // FP32 face-value form: the limiter's first displacement is loaded here and
@@ -473,7 +487,7 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
   // registers and load each slot's displacement when it is used)
   constexpr bool kPipe = kFaceVals && sizeof(T) == 4;
   T dnext[kPipe ? 3 : 1];
-  if constexpr (kPipe) load_disp(m, ent[0], dnext);
+  if constexpr (kPipe) load_geo_disp(m, ent[0] < 0 ? 0 : ent[0] >> 1, ent[0] & 1, dnext);
 #pragma unroll
   for (int s = 0; s < kMaxSlots; ++s) {
     const bool ok = ent[s] >= 0;
@@ -542,15 +556,13 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
     if constexpr (kPipe) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) disp[d] = dnext[d];
-      if (s + 1 < kMaxSlots) load_disp(m, ent[s + 1], dnext);
+      if (s + 1 < kMaxSlots) load_geo_disp(m, ent[s + 1] < 0 ? 0 : ent[s + 1] >> 1, ent[s + 1] & 1, dnext);
 #pragma unroll
       for (int d = 0; d < 3; ++d) dist += disp[d] * disp[d];
     } else {
+      load_geo_disp(m, e >> 1, e & 1, disp);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        disp[d] = static_cast<const T*>(m.fdx)[(size_t)((e & 1) * 3 + d) * nf + (e >> 1)];
-        dist += disp[d] * disp[d];
-      }
+      for (int d = 0; d < 3; ++d) dist += disp[d] * disp[d];
     }
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -585,7 +597,9 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
 
 // Waves per SIMD the FP64 second-order gradient kernels are register-budgeted
 // for: the compiler's own choice can land just over the 168 registers that
-// allow 3 (then 2 waves run); other forms no constraint.
+// allow 3 (then 2 waves run); other forms no constraint (the FP32 viscous
+// kernel runs 5 waves at 84 registers; a 6-wave budget spills and was 14%
+// slower on C3).
 template <bool kSecondOrder, class T>
 constexpr int kGradWaves = sizeof(T) == 8 && kSecondOrder ? 3 : 1;
 
@@ -1603,8 +1617,8 @@ struct Layout {
   std::vector<int> dev_of, org_of, tile_cptr, tptr, tfaces, slot, tslot, fl, fr;
   std::vector<double> fn, ffr, fx, fwall, vol, xc;
   std::vector<signed char> fkind;
-  // geometry records of the internal faces (see dedupe_geometry): the record
-  // of each internal face, and kGeoVals FP64 values per record
+  // geometry records of the faces (see dedupe_geometry): the record of each
+  // face, and kGeoVals FP64 values per record
   std::vector<int> fgeo;
   std::vector<double> geo;
 };
@@ -1638,7 +1652,7 @@ ma_status validate_layout(const Layout& L) {
   if (tile_cptr.front() != 0 || tile_cptr.back() != nc) return ma::fail(MA_ERR_INVALID, "layout: tile cells");
   {
     const size_t ng = L.geo.size() / kGeoVals;
-    if ((int)L.fgeo.size() != nif || L.geo.size() % kGeoVals || (nif > 0 && ng == 0))
+    if ((int)L.fgeo.size() != nf || L.geo.size() % kGeoVals || (nf > 0 && ng == 0))
       return ma::fail(MA_ERR_INVALID, "layout: geometry records");
     for (int x : L.fgeo)
       if (x < 0 || (size_t)x >= ng) return ma::fail(MA_ERR_INVALID, "layout: geometry record index");
@@ -1711,7 +1725,7 @@ void merge_layouts(std::vector<Layout>& parts, Layout& M, std::vector<int>& tile
   M.fx.assign((size_t)3 * nf, 0.0);
   M.vol.assign(nc, 0.0);
   M.xc.assign((size_t)3 * nc, 0.0);
-  M.fgeo.assign(M.nif, 0);
+  M.fgeo.assign(M.nf, 0);
   M.geo.clear();
   tile_case.assign(M.ntiles, 0);
   case_tiles.assign(n + 1, 0);
@@ -1769,20 +1783,22 @@ void merge_layouts(std::vector<Layout>& parts, Layout& M, std::vector<int>& tile
     }
     {  // this part's geometry records follow the earlier parts'
       const int r0 = (int)(M.geo.size() / kGeoVals);
-      for (int f = 0; f < lnif; ++f) M.fgeo[io[k] + f] = r0 + L.fgeo[f];
+      for (int f = 0; f < lnf; ++f) M.fgeo[gf(f)] = r0 + L.fgeo[f];
       M.geo.insert(M.geo.end(), L.geo.begin(), L.geo.end());
     }
     Layout().swap_out(L);  // release the part as soon as it is merged
   }
 }
 
-// Geometry records of the internal faces, which the flux kernel reads: each
-// face's area vector and both displacements x_f - x_left, x_f - x_right
-// (FP64, the expressions the displacement planes use), with faces whose nine
-// values have the same bits sharing one record.  Box meshes repeat few
+// Geometry records of the faces, which the flux and gradient kernels read:
+// each face's area vector and both displacements x_f - x_left, x_f - x_right
+// (FP64, the expressions the displacement planes use; the right one zero for
+// a boundary face), with faces whose nine values have the same bits sharing
+// one record.  Box meshes repeat few
 // records over many faces (the 256 x 128 x 64 stretched plate: 7,534 records
 // for 6.2M internal faces; a uniform box: 3), so the kernel reads a 4-byte
-// record index per face instead of 9 values, and the records stay in L2.
+// record index per face (the gradient kernel per slot) instead of up to 9
+// values, and the records stay in L2.
 // When records repeat fewer than kGeoReuse times on average, every face gets
 // its own record (record f for face f).
 constexpr int kGeoReuse = 8;
@@ -1800,21 +1816,21 @@ struct GeoKeyHash {
 };
 
 void dedupe_geometry(Layout& L) {
-  const int nif = L.nif, nf = L.nf, nc = L.nc;
+  const int nf = L.nf, nc = L.nc;
   auto record = [&](int f, double* r) {
     for (int d = 0; d < 3; ++d) {
       r[d] = L.fn[(size_t)d * nf + f];
       r[3 + d] = L.fx[(size_t)d * nf + f] - L.xc[(size_t)d * nc + L.fl[f]];
-      r[6 + d] = L.fx[(size_t)d * nf + f] - L.xc[(size_t)d * nc + L.fr[f]];
+      r[6 + d] = L.fr[f] >= 0 ? L.fx[(size_t)d * nf + f] - L.xc[(size_t)d * nc + L.fr[f]] : 0.0;
     }
   };
-  L.fgeo.assign((size_t)nif, 0);
+  L.fgeo.assign((size_t)nf, 0);
   L.geo.clear();
-  const size_t cap = std::min<size_t>((size_t)nif / kGeoReuse, size_t(1) << 20);
+  const size_t cap = std::min<size_t>((size_t)nf / kGeoReuse, size_t(1) << 20);
   std::unordered_map<std::array<unsigned long long, kGeoVals>, int, GeoKeyHash> ids;
   ids.reserve(std::min<size_t>(cap, 1 << 16) * 2);
-  bool shared = nif >= kGeoReuse;
-  for (int f = 0; shared && f < nif; ++f) {
+  bool shared = nf >= kGeoReuse;
+  for (int f = 0; shared && f < nf; ++f) {
     double r[kGeoVals];
     record(f, r);
     std::array<unsigned long long, kGeoVals> key;
@@ -1827,8 +1843,8 @@ void dedupe_geometry(Layout& L) {
     L.fgeo[f] = it.first->second;
   }
   if (shared) return;
-  L.geo.assign((size_t)kGeoVals * nif, 0.0);
-  for (int f = 0; f < nif; ++f) {
+  L.geo.assign((size_t)kGeoVals * nf, 0.0);
+  for (int f = 0; f < nf; ++f) {
     record(f, &L.geo[(size_t)kGeoVals * f]);
     L.fgeo[f] = f;
   }
@@ -2318,6 +2334,17 @@ ma_status build_planned(ma_solver* s, const RunPlan& P) {
     if (st) return st;
     m.gtab = dg;
     s->t.fgid = dfg;
+    int* dsg = nullptr;
+    if (s->counting) {
+      st = s->alloc(&dsg, (size_t)kMaxSlots * nc);
+    } else {
+      std::vector<int> h((size_t)kMaxSlots * nc, -1);
+      for (size_t i = 0; i < h.size(); ++i)
+        if (L.slot[i] >= 0) h[i] = (L.fgeo[L.slot[i] >> 1] << 1) | (L.slot[i] & 1);
+      st = upload(s, &dsg, h);
+    }
+    if (st) return st;
+    m.sgeo = dsg;
   }
 
   // per-case parameters
