@@ -414,7 +414,7 @@ def bench_served(args, M, stream, copy_gbs) -> dict:
     first submission); latency = completion - submission."""
     import numpy as np
     srv = M.Server(hbm_budget_bytes=args.hbm_budget, max_active=args.max_active,
-                   setup_threads=args.setup_threads)
+                   setup_threads=args.setup_threads, steps_in_flight=args.steps_in_flight)
     # one small case first, so that module loading is not in the first request's latency
     warm = stream[0][1].replace(nx=8, ny=4, nz=2, ntimesteps=2)
     wid = srv.submit(warm)
@@ -504,6 +504,7 @@ def bench_served(args, M, stream, copy_gbs) -> dict:
               "physics": "Roe, Green-Gauss, Venkat, 2nd order; half viscous flat plate, half slip wall",
               "precision": "fp32 flux/gradient/limiter, fp64 state",
               "max_active": args.max_active or 16, "setup_threads": args.setup_threads,
+              "steps_in_flight": args.steps_in_flight or 2,
               "layout_plans": {"reused": plan_cache["hits"], "planned": plan_cache["misses"]},
               "region_bytes": st0["region_size"],
               "peak_region_bytes_used": int(peak_used),
@@ -696,6 +697,8 @@ def main() -> int:
     ap.add_argument("--hbm-budget", type=float, default=200e9, help="served workload: device bytes for cases")
     ap.add_argument("--setup-threads", type=int, default=14,
                     help="served workload: host threads generating and planning requests")
+    ap.add_argument("--steps-in-flight", type=int, default=0,
+                    help="served workload: RK4 steps queued ahead per scheme group (0 = the server's default, 2)")
     ap.add_argument("--wg-per-cu", type=int, default=None,
                     help="resident workgroups per CU for the flux kernel (0: one per tile)")
     ap.add_argument("--copy-ceiling", action="store_true",

@@ -231,6 +231,9 @@ struct ma_server {
   std::deque<Request*> queue;        // submitted, not yet taken by a setup thread
   std::deque<Request*> incoming;     // built, not yet taken by the scheduler
   std::deque<Request*> done_unpolled;
+  std::deque<Request*> completing;   // last step done, state not yet copied back (finisher thread)
+  std::condition_variable cv_complete;
+  bool sched_done = false;           // the scheduler has returned: the finisher drains and stops
   std::map<long long, std::unique_ptr<Request>> all;
   long long next_id = 1;
   long long admit_next = 1;  // admission follows submission order
@@ -239,7 +242,7 @@ struct ma_server {
   long long finished = 0;
   bool stop = false;
   std::vector<std::thread> setup;
-  std::thread sched;
+  std::thread sched, finish;
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
@@ -387,6 +390,32 @@ void complete(ma_server* S, Request* r) {
   mark_finished(S, r);
 }
 
+// The scheduler's side of a finished case: the finisher thread copies the
+// state back and frees the case, so that the scheduler goes on queueing the
+// other cases' steps meanwhile.  The scheduler does not touch r afterwards.
+void hand_off(ma_server* S, Request* r) {
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->completing.push_back(r);
+  }
+  S->cv_complete.notify_one();
+}
+
+void finisher(ma_server* S) {
+  (void)hipSetDevice(S->device);
+  for (;;) {
+    Request* r = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv_complete.wait(lk, [&] { return S->sched_done || !S->completing.empty(); });
+      if (S->completing.empty()) return;  // the scheduler is gone and nothing is left
+      r = S->completing.front();
+      S->completing.pop_front();
+    }
+    complete(S, r);
+  }
+}
+
 void fail_group(ma_server* S, Group& g, const std::string& why) {
   (void)hipStreamSynchronize(g.stream);
   std::vector<Request*> all(g.act);
@@ -400,7 +429,7 @@ void fail_group(ma_server* S, Group& g, const std::string& why) {
   for (Request* r : all) {
     r->status = MA_ERR_DEVICE;
     r->error = why;
-    complete(S, r);
+    hand_off(S, r);
   }
 }
 
@@ -459,7 +488,7 @@ void scheduler(ma_server* S) {
     for (Request* r : joining) {
       progress = true;
       if (r->total == 0) {
-        complete(S, r);  // the initial state is the result
+        hand_off(S, r);  // the initial state is the result
         continue;
       }
       Group* g = nullptr;
@@ -476,7 +505,7 @@ void scheduler(ma_server* S) {
           if (ng->stream) (void)hipStreamDestroy(ng->stream);
           r->status = st;
           r->error = ma::last_error();
-          complete(S, r);
+          hand_off(S, r);
           continue;
         }
         groups.push_back(std::move(ng));
@@ -502,7 +531,7 @@ void scheduler(ma_server* S) {
         for (Request* r : st.finishing) {
           ma::solver_set_progress(r->solver, r->total);
           r->done = r->total;
-          complete(S, r);  // last use of r
+          hand_off(S, r);  // last use of r here
         }
         progress = true;
       }
@@ -599,6 +628,7 @@ extern "C" ma_status ma_server_create(const ma_server_config* cfg, ma_server** o
   try {
     for (int i = 0; i < c.setup_threads; ++i) S->setup.emplace_back(setup_worker, S);
     S->sched = std::thread(scheduler, S);
+    S->finish = std::thread(finisher, S);
   } catch (const std::exception&) {
     ma_server_destroy(S);
     return ma::fail(MA_ERR_DEVICE, "ma_server_create: could not start threads");
@@ -712,6 +742,12 @@ extern "C" void ma_server_destroy(ma_server* S) {
   for (auto& t : S->setup)
     if (t.joinable()) t.join();
   if (S->sched.joinable()) S->sched.join();
+  {  // the finisher completes what the scheduler handed it, then stops
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->sched_done = true;
+  }
+  S->cv_complete.notify_all();
+  if (S->finish.joinable()) S->finish.join();
   // built after the scheduler stopped
   for (Request* r : S->incoming) retire(S, r);
   S->incoming.clear();
