@@ -136,8 +136,49 @@ MA_HD void tangent_binormal_t(const T* a, T* t, T* b) {
 
 MA_HD void tangent_binormal(const double* a, double* t, double* b) { tangent_binormal_t<double>(a, t, b); }
 
+// An area vector along an axis -- two components +-0, the third of magnitude
+// m in [2^-500, 2^500] (FP32: [2^-60, 2^60]) -- makes every rounding of the
+// general path exact: sqrt(RN(x^2)) = |x| in binary floating point with
+// rounding to nearest (no overflow or underflow), x / |x| = +-1, +-0 / y =
+// +-0 for y > 0, and t has unit length.  Its frame is therefore built with
+// selects (axis_frame_t), the same bits as tangent_binormal_t + make_frame_t,
+// signed zeros included (b keeps the general formula).  Box meshes have only
+// such faces.
+template <class T>
+MA_HD bool is_axis_area(const T* a) {
+  const bool z0 = a[0] == T(0), z1 = a[1] == T(0), z2 = a[2] == T(0);
+  const T m = fabs(!z0 ? a[0] : (!z1 ? a[1] : a[2]));
+  const T lo = T(sizeof(T) == 8 ? 0x1p-500 : 0x1p-60), hi = T(sizeof(T) == 8 ? 0x1p500 : 0x1p60);
+  return int(z0) + int(z1) + int(z2) == 2 && m >= lo && m <= hi;
+}
+
+template <class T>
+MA_HD FaceFrameT<T> axis_frame_t(const T* a) {
+  const bool z0 = a[0] == T(0), z1 = a[1] == T(0);
+  const int i1 = !z0 ? 0 : (!z1 ? 1 : 2);  // tangent_binormal_t's i1: the largest |a_i|
+  const T p = i1 == 0 ? a[0] : (i1 == 1 ? a[1] : a[2]);
+  const T q = i1 == 0 ? a[2] : (i1 == 1 ? a[0] : a[1]);  // a[i1 + 2], +-0: q / |p| = q
+  const T tq = copysign(T(1), -p);                       // -p / |p|
+  FaceFrameT<T> f;
+  f.t[0] = i1 == 0 ? q : (i1 == 1 ? tq : T(0.0));
+  f.t[1] = i1 == 0 ? T(0.0) : (i1 == 1 ? q : tq);
+  f.t[2] = i1 == 0 ? tq : (i1 == 1 ? T(0.0) : q);
+  f.b[0] = a[1] * f.t[2] - f.t[1] * a[2];
+  f.b[1] = -a[0] * f.t[2] + f.t[0] * a[2];
+  f.b[2] = a[0] * f.t[1] - f.t[0] * a[1];
+  for (int d = 0; d < 3; ++d) {
+    f.n[d] = a[d];
+    f.nu[d] = a[d] == T(0) ? a[d] : copysign(T(1), a[d]);      // a / |a|
+    f.tu[d] = f.t[d];                                           // t / 1
+    f.bu[d] = f.b[d] == T(0) ? f.b[d] : copysign(T(1), f.b[d]);  // b / |b|, |b| = |p|
+  }
+  f.nn = fabs(p);
+  return f;
+}
+
 template <class T>
 MA_HD FaceFrameT<T> frame_from_area_t(const T* a) {
+  if (is_axis_area<T>(a)) return axis_frame_t<T>(a);
   T t[3], b[3];
   tangent_binormal_t<T>(a, t, b);
   return make_frame_t<T>(a, t, b);
