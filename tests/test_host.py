@@ -328,3 +328,25 @@ def test_bench_algorithmic_bytes_match_the_stated_figures():
     assert bench.algorithmic_bytes("k_prim", nc, nif, nbf, True) == 120 * nc
     assert bench.path_bytes_per_step(nc, nif, nbf, True, True) == 4 * (824 * nc + 168 * F)
     assert abs(bench.path_bytes_per_step(nc, nif, nbf, True, True) / (4 * nc) - 1333) < 1
+
+
+def test_axis_aligned_face_frame_equals_general_path(tmp_path):
+    """The kernels build the Roe face frame of an axis-aligned area vector with
+    selects (axis_frame_t, physics.hpp); it must equal the general path's
+    divisions and square roots bit for bit, FP64 and FP32 (host IEEE
+    arithmetic), and refuse vectors that are not along an axis."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    here = os.path.dirname(os.path.abspath(__file__))
+    exe = str(tmp_path / "axis_frame_check")
+    subprocess.run([hipcc, "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "--offload-arch=gfx950",
+                    "-I", os.path.join(here, "..", "miniaero_amd", "csrc"),
+                    os.path.join(here, "host_checks", "axis_frame_check.cpp"), "-o", exe],
+                   check=True, capture_output=True)
+    r = subprocess.run([exe, "2000000"], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout
+    assert "mismatches=0" in r.stdout and "fast=0" not in r.stdout
