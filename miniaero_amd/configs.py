@@ -59,7 +59,7 @@ def flat_plate_c2(**kw) -> Options:
 
 
 def flat_plate_c3(**kw) -> Options:
-    """The C2 physics on 512x256x64, started from the freestream with a seeded
+    """The 256x128x64 flat plate's physics on 512x256x64, started from the freestream with a seeded
     +-1% perturbation of density and velocity, 500 steps, flux/gradient/limiter
     arithmetic in FP32 with FP64 state and sums."""
     return flat_plate_c2(nx=512, ny=256, nz=64, ntimesteps=500, ic_kind=2, perturb=0.01, seed=3,
@@ -92,11 +92,11 @@ def batched_c4(ncases: int = 512, seed: int = 4, **kw) -> list:
 
 
 def served_case(viscous: bool, mach: float, angle_deg: float, cells: int, steps: int) -> Options:
-    """One request of the served stream: the C2 flat-plate box (4:2:1 cells,
+    """One request of the served stream: the stretched flat-plate box (4:2:1 cells,
     m = round((cells/8)^(1/3)), nx = 4m, ny = 2m, nz = m) with a freestream
     at the given Mach number and inflow angle, T 300 K, Re 1e5 per metre,
     CFL 0.5, second order with Venkat, FP32 flux/gradient/limiter arithmetic
-    and FP64 state.  Viscous: the C2 boundaries and y-stretching.  Inviscid:
+    and FP64 state.  Viscous: the 256x128x64 plate's boundaries and y-stretching.  Inviscid:
     the same box with a slip (Tangent) wall at -y and uniform spacing."""
     m = max(1, int(round((cells / 8.0) ** (1.0 / 3.0))))
     o = flat_plate_c2(nx=4 * m, ny=2 * m, nz=m, ntimesteps=int(steps),

@@ -67,6 +67,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -122,6 +123,9 @@ struct DevMesh {
   // kMaxSlots planes of nc: (record << 1) | side of each slot's face, -1 for
   // an empty slot (the gradient kernel's slot list)
   const int* __restrict__ sgeo;
+  // 1: every face's area vector has at most one nonzero component and every
+  // cell's 1 / vol is finite (box meshes; see grad_cell)
+  int axis;
 };
 
 struct Inflow {
@@ -447,7 +451,9 @@ __global__ __launch_bounds__(kBlock) void k_prim(int nc, const double* __restric
 // Otherwise the record is left in out[kRec] for the caller, which stores a
 // wavefront's records together (store_records).  First order has zero
 // limiters.
-template <bool kSecondOrder, bool kFaceVals, class Limiter, class T>
+// kAxis: 0 the general Green-Gauss loop, 1 the axis-aligned one (the mesh
+// has DevMesh::axis set), 2 whichever m.axis selects at run time.
+template <bool kSecondOrder, bool kFaceVals, class Limiter, class T, int kAxis = 0>
 __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restrict__ nbr, const T* __restrict__ V,
                                           T* __restrict__ rec, T* __restrict__ fv, int c, T* out) {
   const int nc = m.nc;
@@ -483,11 +489,26 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
   }
   // FP32 face-value form: the limiter's first displacement is loaded here and
   // each later one a slot ahead, while the previous slot's limiter arithmetic
-  // runs (C4 k_grad -3%; the other forms lose occupancy to the extra
-  // registers and load each slot's displacement when it is used)
+  // runs (512-case batched Sod workload: gradient kernel -3%; the other forms
+  // lose occupancy to the extra registers and load each slot's displacement
+  // when it is used)
   constexpr bool kPipe = kFaceVals && sizeof(T) == 4;
   T dnext[kPipe ? 3 : 1];
   if constexpr (kPipe) load_geo_disp(m, ent[0] < 0 ? 0 : ent[0] >> 1, ent[0] & 1, dnext);
+  // Axis-aligned faces (every face of a box mesh): for a zero component n_d
+  // of the area vector the Green-Gauss term gg = 0.5 (v_L + v_R) n_d is +-0
+  // or NaN, and then gg / vol and gg * (1 / vol) are the same value (signed
+  // zero or NaN) whenever 1 / vol is finite.  So on a mesh whose area vectors
+  // have at most one nonzero component (DevMesh::axis, checked on the host
+  // with the volumes) a slot needs one division per variable, for that
+  // component, and multiplications for the other two -- the same results, a
+  // third of the divisions.  Used in the FP32 face-value form (512-case
+  // batched Sod workload: gradient kernel -6.5%); the FP32 viscous form
+  // multiplies by the reciprocal
+  // throughout, and in FP64 this loop took the gradient kernels past their
+  // register budget (424 B of spills, 2.3x slower).
+  constexpr bool kDivides = sizeof(T) == 4 && kFaceVals;
+  const bool axis = kDivides && (kAxis == 1 || (kAxis == 2 && m.axis));
 #pragma unroll
   for (int s = 0; s < kMaxSlots; ++s) {
     const bool ok = ent[s] >= 0;
@@ -495,21 +516,43 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
     const int side = e & 1;
     const T* n = an[s];
     const T* vn = vnn[s];
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
+    if (axis) {  // uniform: a property of the mesh
+      const int dn = n[1] != T(0) ? 1 : (n[2] != T(0) ? 2 : 0);  // the nonzero component (0 if none)
+      // n[dn], exactly: the other two are +-0 (a sum instead of a select by
+      // dn, which the compiler would turn into an indexed load from memory;
+      // when all three are zero its sign may differ, and a +-0 term does not
+      // change the sums)
+      const T nd = (n[0] + n[1]) + n[2];
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
         const T vl = side ? vn[k] : vc[k];
         const T vr = side ? vc[k] : vn[k];
-        const T gg = ok ? T(0.5) * (vl + vr) * n[d] : T(0);
-        // FP32 viscous form: one reciprocal per cell (k_grad -6%; in the
-        // face-value scheme it measured 2% slower, so that keeps the division);
-        // FP64: the reference's division per face
-        if (sizeof(T) == 4 && !kFaceVals)
-          g[k][d] += (side ? -gg : gg) * rvol;
-        else
-          g[k][d] += side ? -gg / vol : gg / vol;
+        const T base = T(0.5) * (vl + vr);
+        const T gn = ok ? base * nd : T(0);
+        const T q = side ? -gn / vol : gn / vol;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const T gd = ok ? base * n[d] : T(0);
+          g[k][d] += d == dn ? q : (side ? -gd * rvol : gd * rvol);
+        }
       }
+    } else {
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          const T vl = side ? vn[k] : vc[k];
+          const T vr = side ? vc[k] : vn[k];
+          const T gg = ok ? T(0.5) * (vl + vr) * n[d] : T(0);
+          // FP32 viscous form: one reciprocal per cell (k_grad -6%; in the
+          // face-value scheme it measured 2% slower, so that keeps the
+          // division); FP64: the reference's division per face
+          if (sizeof(T) == 4 && !kFaceVals)
+            g[k][d] += (side ? -gg : gg) * rvol;
+          else
+            g[k][d] += side ? -gg / vol : gg / vol;
+        }
+    }
     if (kSecondOrder) {
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
@@ -599,11 +642,11 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
 // for: the compiler's own choice can land just over the 168 registers that
 // allow 3 (then 2 waves run); other forms no constraint (the FP32 viscous
 // kernel runs 5 waves at 84 registers; a 6-wave budget spills and was 14%
-// slower on C3).
+// slower on the 512x256x64 plate).
 template <bool kSecondOrder, class T>
 constexpr int kGradWaves = sizeof(T) == 8 && kSecondOrder ? 3 : 1;
 
-template <bool kSecondOrder, bool kFaceVals, class Limiter, class T>
+template <bool kSecondOrder, bool kFaceVals, class Limiter, class T, int kAxis>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGradWaves<kSecondOrder, T>)))
 void k_grad(DevMesh m, const int* __restrict__ nbr, const T* __restrict__ V, T* __restrict__ rec,
             T* __restrict__ fv, int per_xcd) {
@@ -612,7 +655,7 @@ void k_grad(DevMesh m, const int* __restrict__ nbr, const T* __restrict__ V, T* 
   const int b = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
   const int c = b * kBlock + threadIdx.x;
   T out[kRec];
-  if (c < m.nc) grad_cell<kSecondOrder, kFaceVals, Limiter, T>(m, nbr, V, rec, fv, c, out);
+  if (c < m.nc) grad_cell<kSecondOrder, kFaceVals, Limiter, T, kAxis>(m, nbr, V, rec, fv, c, out);
   if constexpr (!kFaceVals) {
     if (b * kBlock < m.nc) store_records(rec, stage, out, b * kBlock, m.nc);  // uniform over the workgroup
   }
@@ -1095,7 +1138,7 @@ void k_grad_multi(const CaseView* __restrict__ cv, const int* __restrict__ start
   const int c0 = (g - start[k]) * kBlock, c = c0 + threadIdx.x;
   T out[kRec];
   if (c < v.m.nc)
-    grad_cell<kSecondOrder, kFaceVals, Limiter, T>(v.m, v.nbr, static_cast<const T*>(v.V[vsel]),
+    grad_cell<kSecondOrder, kFaceVals, Limiter, T, 2>(v.m, v.nbr, static_cast<const T*>(v.V[vsel]),
                                                    store_grad ? static_cast<T*>(v.grad) : nullptr,
                                                    static_cast<T*>(v.fv), c, out);
   if constexpr (!kFaceVals) {
@@ -1281,7 +1324,8 @@ void k_steps(PersistArgs a, Inv inv, Visc visc) {
           if (b * kBlock >= m.nc) break;  // uniform over the workgroup
           const int c = b * kBlock + threadIdx.x;
           T out[kRec];
-          if (c < m.nc) grad_cell<SO, FV, Lim, T>(m, a.nbr, Vs, rec, static_cast<T*>(a.fv), c, out);
+          if (c < m.nc)
+            grad_cell<SO, FV, Lim, T, 2>(m, a.nbr, Vs, rec, static_cast<T*>(a.fv), c, out);
           if constexpr (!FV) {
             store_records(rec, lds, out, b * kBlock, m.nc);
             __syncthreads();  // the staging rows are reused by the next block
@@ -1621,6 +1665,9 @@ struct Layout {
   // face, and kGeoVals FP64 values per record
   std::vector<int> fgeo;
   std::vector<double> geo;
+  // every area vector has at most one nonzero component and every cell
+  // volume has a finite reciprocal in FP32 and FP64 (DevMesh::axis)
+  bool axis = false;
 };
 
 // Every index a kernel will use, checked on the host before any upload.
@@ -1727,6 +1774,8 @@ void merge_layouts(std::vector<Layout>& parts, Layout& M, std::vector<int>& tile
   M.xc.assign((size_t)3 * nc, 0.0);
   M.fgeo.assign(M.nf, 0);
   M.geo.clear();
+  M.axis = true;
+  for (int k = 0; k < n; ++k) M.axis = M.axis && parts[k].axis;
   tile_case.assign(M.ntiles, 0);
   case_tiles.assign(n + 1, 0);
   bc_case.assign(std::max(1, M.nbf), 0);
@@ -1817,6 +1866,12 @@ struct GeoKeyHash {
 
 void dedupe_geometry(Layout& L) {
   const int nf = L.nf, nc = L.nc;
+  L.axis = true;
+  for (int f = 0; f < nf && L.axis; ++f)
+    L.axis = int(L.fn[f] != 0.0) + int(L.fn[(size_t)nf + f] != 0.0) + int(L.fn[(size_t)2 * nf + f] != 0.0) <= 1;
+  for (int c = 0; c < nc && L.axis; ++c)
+    L.axis = std::fabs(L.vol[c]) >= (double)std::numeric_limits<float>::min() &&
+             std::fabs(L.vol[c]) <= (double)std::numeric_limits<float>::max();
   auto record = [&](int f, double* r) {
     for (int d = 0; d < 3; ++d) {
       r[d] = L.fn[(size_t)d * nf + f];
@@ -2334,6 +2389,7 @@ ma_status build_planned(ma_solver* s, const RunPlan& P) {
     if (st) return st;
     m.gtab = dg;
     s->t.fgid = dfg;
+    m.axis = L.axis ? 1 : 0;
     int* dsg = nullptr;
     if (s->counting) {
       st = s->alloc(&dsg, (size_t)kMaxSlots * nc);
@@ -2472,9 +2528,17 @@ void run_stage(ma_solver* s, int irk) {
     s->launch(SO ? "k_grad_limiter" : "k_grad", [&] {
       const int per = (nb + kXcds - 1) / kXcds;
       const bool store = !FV;
-      hipLaunchKernelGGL((k_grad<SO, FV, Lim, T>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, m, s->nbr,
-                         static_cast<const T*>(Vs), store ? static_cast<T*>(s->grad) : nullptr,
-                         static_cast<T*>(s->fv), per);
+      // the axis-aligned Green-Gauss loop (FP32 face-value scheme) when the
+      // mesh has DevMesh::axis
+      constexpr bool kDiv = sizeof(T) == 4 && FV;
+      if (kDiv && m.axis)
+        hipLaunchKernelGGL((k_grad<SO, FV, Lim, T, kDiv ? 1 : 0>), dim3(per * kXcds), dim3(kBlock), 0, s->stream,
+                           m, s->nbr, static_cast<const T*>(Vs), store ? static_cast<T*>(s->grad) : nullptr,
+                           static_cast<T*>(s->fv), per);
+      else
+        hipLaunchKernelGGL((k_grad<SO, FV, Lim, T, 0>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, m, s->nbr,
+                           static_cast<const T*>(Vs), store ? static_cast<T*>(s->grad) : nullptr,
+                           static_cast<T*>(s->fv), per);
     });
   }
   Visc visc{effective_prandtl(s->opt)};
@@ -3042,7 +3106,7 @@ template <class Lim, class T>
 void stage3_gradients(ma_solver* s) {
   const int nb = nblocks(s->m.nc);
   const int per = (nb + kXcds - 1) / kXcds;
-  hipLaunchKernelGGL((k_grad<true, true, Lim, T>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, s->m, s->nbr,
+  hipLaunchKernelGGL((k_grad<true, true, Lim, T, 0>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, s->m, s->nbr,
                      static_cast<const T*>(s->V2), static_cast<T*>(s->grad), static_cast<T*>(s->fv), per);
 }
 
