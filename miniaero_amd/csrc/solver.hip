@@ -497,18 +497,21 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
   if constexpr (kPipe) load_geo_disp(m, ent[0] < 0 ? 0 : ent[0] >> 1, ent[0] & 1, dnext);
   // Axis-aligned faces (every face of a box mesh): for a zero component n_d
   // of the area vector the Green-Gauss term gg = 0.5 (v_L + v_R) n_d is +-0
-  // or NaN, and then gg / vol and gg * (1 / vol) are the same value (signed
-  // zero or NaN) whenever 1 / vol is finite.  So on a mesh whose area vectors
-  // have at most one nonzero component (DevMesh::axis, checked on the host
-  // with the volumes) a slot needs one division per variable, for that
-  // component, and multiplications for the other two -- the same results, a
-  // third of the divisions.  Used in the FP32 face-value form (512-case
-  // batched Sod workload: gradient kernel -6.5%); the FP32 viscous form
-  // multiplies by the reciprocal
-  // throughout, and in FP64 this loop took the gradient kernels past their
-  // register budget (424 B of spills, 2.3x slower).
-  constexpr bool kDivides = sizeof(T) == 4 && kFaceVals;
-  const bool axis = kDivides && (kAxis == 1 || (kAxis == 2 && m.axis));
+  // or NaN, and so are gg / vol and gg * (1 / vol) (the volume is nonzero and
+  // its reciprocal finite).  Adding +-0 leaves a sum unchanged (it starts at
+  // +0 and is never -0), so on a mesh whose area vectors have at most one
+  // nonzero component (DevMesh::axis, checked on the host with the volumes)
+  // a slot divides only that component: the same results with a third of
+  // the divisions.  FP64 adds gg itself for the other two, with the side's
+  // sign in the area vector (-(a b) = a (-b) and (-x) / vol = -(x / vol)
+  // exactly); FP32 multiplies them by 1 / vol, the form that keeps the
+  // multi-case kernel within its registers.  The multi-case and persistent
+  // kernels (kAxis 2) have both loops and take this one in the FP32
+  // face-value form only: in FP64 both loops together spill.  The FP32
+  // viscous form multiplies by the reciprocal throughout, so there is no
+  // division to save.
+  constexpr bool kAxisForm = sizeof(T) == 4 ? kFaceVals : kAxis == 1;
+  const bool axis = kAxisForm && (kAxis == 1 || (kAxis == 2 && m.axis));
 #pragma unroll
   for (int s = 0; s < kMaxSlots; ++s) {
     const bool ok = ent[s] >= 0;
@@ -516,12 +519,30 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
     const int side = e & 1;
     const T* n = an[s];
     const T* vn = vnn[s];
-    if (axis) {  // uniform: a property of the mesh
+    if (axis && sizeof(T) == 8) {  // uniform: a property of the mesh
+      T sn[3];
+      bool nz[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        sn[d] = side ? -n[d] : n[d];
+        nz[d] = n[d] != T(0);
+      }
+      // the nonzero component, exactly: the other two are +-0 (when all
+      // three are zero its sign may differ, and a +-0 term changes no sum)
+      const T snd = (sn[0] + sn[1]) + sn[2];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const T vl = side ? vn[k] : vc[k];
+        const T vr = side ? vc[k] : vn[k];
+        const T base = ok ? T(0.5) * (vl + vr) : T(0);
+        const T q = (base * snd) / vol;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) g[k][d] += nz[d] ? q : base * sn[d];
+      }
+    } else if (axis) {
       const int dn = n[1] != T(0) ? 1 : (n[2] != T(0) ? 2 : 0);  // the nonzero component (0 if none)
-      // n[dn], exactly: the other two are +-0 (a sum instead of a select by
-      // dn, which the compiler would turn into an indexed load from memory;
-      // when all three are zero its sign may differ, and a +-0 term does not
-      // change the sums)
+      // n[dn], exactly (a sum instead of a select by dn, which the compiler
+      // would turn into an indexed load from memory)
       const T nd = (n[0] + n[1]) + n[2];
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
@@ -642,24 +663,11 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
 // for: the compiler's own choice can land just over the 168 registers that
 // allow 3 (then 2 waves run); other forms no constraint (the FP32 viscous
 // kernel runs 5 waves at 84 registers; a 6-wave budget spills and was 14%
-// slower on the 512x256x64 plate).
+// slower on the 512x256x64 plate).  The axis-aligned FP64 viscous kernel
+// spills 12 bytes per lane at 3 waves and is still 10% faster than at 2
+// waves without spills (0.222 against 0.245 ms on the 256x128x64 plate).
 template <bool kSecondOrder, class T>
 constexpr int kGradWaves = sizeof(T) == 8 && kSecondOrder ? 3 : 1;
-
-template <bool kSecondOrder, bool kFaceVals, class Limiter, class T, int kAxis>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGradWaves<kSecondOrder, T>)))
-void k_grad(DevMesh m, const int* __restrict__ nbr, const T* __restrict__ V, T* __restrict__ rec,
-            T* __restrict__ fv, int per_xcd) {
-  __shared__ T stage[kFaceVals ? 1 : kBlock * kStageRow<T>];
-  // XCD-aware: each XCD takes a contiguous run of blocks (see k_flux)
-  const int b = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
-  const int c = b * kBlock + threadIdx.x;
-  T out[kRec];
-  if (c < m.nc) grad_cell<kSecondOrder, kFaceVals, Limiter, T, kAxis>(m, nbr, V, rec, fv, c, out);
-  if constexpr (!kFaceVals) {
-    if (b * kBlock < m.nc) store_records(rec, stage, out, b * kBlock, m.nc);  // uniform over the workgroup
-  }
-}
 
 // Flux of internal face f between cells L and Rr: compute_face_flux's body
 // (Flux.h:89-160) without the scatter -- primitives of both cells, the
@@ -777,6 +785,30 @@ __global__ __launch_bounds__(kBlock) void k_bc(DevMesh m, const T* __restrict__ 
                                                const int* __restrict__ bc_case, T* __restrict__ bc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m.nf - m.nif) bc_face<Inv, ma::ReferenceBoundaries>(m, V, inv, prandtl, inflows, bc_case, bc, i);
+}
+
+// Workgroups past the per_xcd * kXcds cell blocks take the boundary faces,
+// k_bc's work: it reads the same stage primitives, so one launch does both.
+template <bool kSecondOrder, bool kFaceVals, class Limiter, class T, int kAxis, class Inv>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGradWaves<kSecondOrder, T>)))
+void k_grad(DevMesh m, const int* __restrict__ nbr, const T* __restrict__ V, T* __restrict__ rec,
+            T* __restrict__ fv, int per_xcd, Inv inv, double prandtl, const Inflow* __restrict__ inflows,
+            const int* __restrict__ bc_case, T* __restrict__ bc) {
+  __shared__ T stage[kFaceVals ? 1 : kBlock * kStageRow<T>];
+  const int ncb = per_xcd * kXcds;
+  if ((int)blockIdx.x >= ncb) {  // a boundary-face block (uniform over the workgroup)
+    const int i = ((int)blockIdx.x - ncb) * kBlock + threadIdx.x;
+    if (i < m.nf - m.nif) bc_face<Inv, ma::ReferenceBoundaries>(m, V, inv, prandtl, inflows, bc_case, bc, i);
+    return;
+  }
+  // XCD-aware: each XCD takes a contiguous run of blocks (see k_flux)
+  const int b = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  const int c = b * kBlock + threadIdx.x;
+  T out[kRec];
+  if (c < m.nc) grad_cell<kSecondOrder, kFaceVals, Limiter, T, kAxis>(m, nbr, V, rec, fv, c, out);
+  if constexpr (!kFaceVals) {
+    if (b * kBlock < m.nc) store_records(rec, stage, out, b * kBlock, m.nc);  // uniform over the workgroup
+  }
 }
 
 // Residual and RK accumulate of one tile of at most kTile cells:
@@ -2524,29 +2556,34 @@ void run_stage(ma_solver* s, int irk) {
   // gradients and limiters are then not stored (the getter computes the last
   // stage's from V2, see fill_gradients)
   constexpr bool FV = SO && !Visc::isViscous;
+  Visc visc{effective_prandtl(s->opt)};
+  const Inv inv{};
+  const int nbf = m.nf - m.nif;
   if (SO || Visc::isViscous) {
+    // the gradient kernel; its workgroups past the cell blocks compute the
+    // boundary faces (k_bc's work, timed with it)
     s->launch(SO ? "k_grad_limiter" : "k_grad", [&] {
       const int per = (nb + kXcds - 1) / kXcds;
       const bool store = !FV;
-      // the axis-aligned Green-Gauss loop (FP32 face-value scheme) when the
-      // mesh has DevMesh::axis
-      constexpr bool kDiv = sizeof(T) == 4 && FV;
+      const dim3 grid(per * kXcds + (nbf > 0 ? nblocks(nbf) : 0));
+      // the axis-aligned Green-Gauss loop when the mesh has DevMesh::axis
+      // (every form that divides by the volume)
+      constexpr bool kDiv = sizeof(T) == 8 || FV;
       if (kDiv && m.axis)
-        hipLaunchKernelGGL((k_grad<SO, FV, Lim, T, kDiv ? 1 : 0>), dim3(per * kXcds), dim3(kBlock), 0, s->stream,
-                           m, s->nbr, static_cast<const T*>(Vs), store ? static_cast<T*>(s->grad) : nullptr,
-                           static_cast<T*>(s->fv), per);
+        hipLaunchKernelGGL((k_grad<SO, FV, Lim, T, kDiv ? 1 : 0, Inv>), grid, dim3(kBlock), 0, s->stream, m,
+                           s->nbr, static_cast<const T*>(Vs), store ? static_cast<T*>(s->grad) : nullptr,
+                           static_cast<T*>(s->fv), per, inv, visc.prandtl, s->inflow_dev, s->bc_case,
+                           static_cast<T*>(s->bc));
       else
-        hipLaunchKernelGGL((k_grad<SO, FV, Lim, T, 0>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, m, s->nbr,
+        hipLaunchKernelGGL((k_grad<SO, FV, Lim, T, 0, Inv>), grid, dim3(kBlock), 0, s->stream, m, s->nbr,
                            static_cast<const T*>(Vs), store ? static_cast<T*>(s->grad) : nullptr,
-                           static_cast<T*>(s->fv), per);
+                           static_cast<T*>(s->fv), per, inv, visc.prandtl, s->inflow_dev, s->bc_case,
+                           static_cast<T*>(s->bc));
     });
   }
-  Visc visc{effective_prandtl(s->opt)};
-  const Inv inv{};
   // Stage 0 accumulates from sol_n (sol_np1 == sol_n at the start of a step).
   const double* acc_in = irk == 0 ? Un : Uacc;
-  const int nbf = m.nf - m.nif;
-  if (nbf > 0) {
+  if (nbf > 0 && !(SO || Visc::isViscous)) {
     s->launch("k_bc", [&] {
       hipLaunchKernelGGL((k_bc<Inv, T>), dim3(nblocks(nbf)), dim3(kBlock), 0, s->stream, m,
                          static_cast<const T*>(Vs), inv, visc.prandtl, s->inflow_dev, s->bc_case,
@@ -3106,8 +3143,10 @@ template <class Lim, class T>
 void stage3_gradients(ma_solver* s) {
   const int nb = nblocks(s->m.nc);
   const int per = (nb + kXcds - 1) / kXcds;
-  hipLaunchKernelGGL((k_grad<true, true, Lim, T, 0>), dim3(per * kXcds), dim3(kBlock), 0, s->stream, s->m, s->nbr,
-                     static_cast<const T*>(s->V2), static_cast<T*>(s->grad), static_cast<T*>(s->fv), per);
+  // no boundary-face blocks: the grid is the cell blocks alone
+  hipLaunchKernelGGL((k_grad<true, true, Lim, T, 0, Inviscid>), dim3(per * kXcds), dim3(kBlock), 0, s->stream,
+                     s->m, s->nbr, static_cast<const T*>(s->V2), static_cast<T*>(s->grad), static_cast<T*>(s->fv),
+                     per, Inviscid{}, 0.0, nullptr, nullptr, nullptr);
 }
 
 // Inviscid second order stores no gradients or limiters while stepping.  The
