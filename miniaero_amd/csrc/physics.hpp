@@ -388,13 +388,15 @@ struct NewtonianViscousFlux {
     T div = 0;
     for (int i = 0; i < 5; ++i) vflux[i] = T(0.0);
     for (int i = 0; i < 3; ++i) div += g[i + 1][i];
+    // div * delta / 3 with the Kronecker delta: div / 3 on the diagonal; off
+    // it (div * 0) / 3, which is div * 0 itself (+-0 / 3 = +-0, NaN / 3 =
+    // NaN), so the division is left out there
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        const T delta = (i == j) ? 1 : 0;
         const T S = T(0.5) * (g[i + 1][j] + g[j + 1][i]);
-        const T tij = S - div * delta / T(3.);
+        const T tij = S - (i == j ? div / T(3.) : div * T(0));
         vflux[1 + i] += (T(2) * mu * tij) * a[j];
         vflux[4] += (T(2) * mu * tij) * V[i + 1] * a[j];
       }
