@@ -16,7 +16,8 @@
 //            inviscid second order: also the cell's extrapolated values
 //            at its faces, which k_flux then reads instead of V, grad, phi
 //   k_bc     boundary-face contributions (Roe of the ghost state    (per boundary face)
-//            [+ no-slip viscous flux])
+//            [+ no-slip viscous flux]); when k_grad runs, its extra
+//            workgroups do this in the same launch
 //   k_flux   per tile of 256 cells: every internal face touching    (per cell tile)
 //            the tile evaluated once into LDS (Roe [+ reconstruction]
 //            [+ viscous]), then per cell: sum of face and boundary
