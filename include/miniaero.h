@@ -77,7 +77,8 @@ typedef struct ma_options {
   unsigned long long seed;   /* MA_IC_PERTURBED_INFLOW: generator key */
   int precision;             /* MA_PRECISION_* */
   /* Flux tiles: cells per workgroup as an i x j x k brick of a lattice mesh
-   * (extents >= 1, product <= 256); all 0 = 4 x 8 x 8; tile_shape[0] < 0 =
+   * (extents >= 1, product <= 256); all 0 = 4 x 8 x 8, or 4 x 4 x 4 when the
+   * run has fewer cells than 256 per compute unit; tile_shape[0] < 0 =
    * 256 consecutive cells of the caller's order.  Meshes that are not a
    * lattice always use consecutive cells.  Results do not depend on it. */
   int tile_shape[3];

@@ -1402,6 +1402,12 @@ struct TileShape {
   int b[3];
 };
 constexpr TileShape kDefaultShape = {{4, 8, 8}};
+// The default for a run with fewer cells than one default tile per CU: with
+// so few tiles a step's time is each tile's latency, and a quarter of the
+// faces per 256-thread workgroup shortens it (64x16x16 first-order Sod,
+// 16k cells: 77 -> 53 us per step as one graph, 81 -> 58 with a launch per
+// kernel; 2x4x4 and 2x2x4 tiles 56 us, 2x2x2 93 us)
+constexpr TileShape kSmallRunShape = {{4, 4, 4}};
 
 bool valid_shape(const TileShape& t) {
   if (t.b[0] == 0 && t.b[1] == 0 && t.b[2] == 0) return true;
@@ -2932,7 +2938,7 @@ ma_status autotune(int ncases, const ma_mesh_data* const* mds, const ma_options*
                    double* best_ms) {
   static const TileShape cand[] = {{{4, 8, 8}}, {{8, 8, 4}}, {{8, 4, 8}}, {{4, 4, 16}}, {{16, 4, 4}},
                                    {{2, 8, 16}}, {{8, 2, 16}}, {{4, 16, 4}}, {{2, 16, 8}}, {{4, 4, 8}},
-                                   {{0, 0, 0}}};
+                                   {{4, 4, 4}}, {{0, 0, 0}}};
   static const int wgs[] = {1, 2, 3, 4};
   long long total = 0;
   for (int k = 0; k < ncases; ++k) total += mds[k]->num_owned_cells;
@@ -2999,6 +3005,15 @@ ma_status create_run(int ncases, const ma_mesh_data* const* mds, const ma_option
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return ma::fail(MA_ERR_DEVICE, "no HIP device available");
+  if (!force && !o0.autotune && o0.tile_shape[0] == 0 && o0.tile_shape[1] == 0 && o0.tile_shape[2] == 0) {
+    long long cells = 0;
+    for (int k = 0; k < ncases; ++k) cells += mds[k]->num_owned_cells;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        cells < (long long)cus * kTile)
+      tune.shape = kSmallRunShape;
+  }
   double tuned_ms = 0.0;
   if (!force && o0.autotune) {
     ma_status st = MA_OK;

@@ -39,7 +39,8 @@ class Options:
     perturb: float = 0.0
     seed: int = 0
     precision: int = 0        # 0 FP64, 1 FP32 flux/gradient/limiter arithmetic
-    tile_shape: tuple = (0, 0, 0)   # flux tile brick (i, j, k); zeros = 4x8x8, (-1, ..) = consecutive
+    tile_shape: tuple = (0, 0, 0)   # flux tile brick (i, j, k); zeros = 4x8x8 (4x4x4 below 256 cells
+                                    # per CU), (-1, ..) = consecutive
     autotune: int = 0         # 1: pick the fastest tile shape / resident workgroups at creation
     workgroups_per_cu: int = 0  # flux kernel: resident workgroups per CU (0: one per tile)
 

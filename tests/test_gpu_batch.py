@@ -269,6 +269,30 @@ def test_tile_shape_does_not_change_results(shape, wg):
     assert np.array_equal(out[0], out[1])
 
 
+def test_small_run_default_tiles():
+    """A run with fewer cells than one 4x8x8 tile per compute unit gets 4x4x4
+    tiles by default, a larger one 4x8x8; either gives the bits of the other
+    shape, so the choice is a matter of speed only."""
+    for n, want in ((16, (4, 4, 4)), (64, (4, 8, 8))):
+        kw = plate(nx=n, ny=32, nz=32, y_stretch=1.05, bc_kind=(3, 2, 2, 2, 1, 4), dt=0.0, cfl=0.5)
+        md, opts = make_case(kw)
+        out = []
+        for o in (opts, opts.replace(tile_shape=(4, 8, 8) if want == (4, 4, 4) else (4, 4, 4))):
+            s = M.TimeSolverExplicitRK4(md, o)
+            try:
+                s.initialize()
+                s.step(4)
+                out.append(s.state())
+                if len(out) == 1:
+                    got, _ = s.tile_shape()
+            finally:
+                s.close()
+        print(f"[gpu-batch] {n * 32 * 32} cells: default tiles {got}, bitwise={np.array_equal(out[0], out[1])}",
+              flush=True)
+        assert got == want
+        assert np.array_equal(out[0], out[1])
+
+
 @pytest.mark.parametrize("shape,wg", [((-1, 0, 0), 0), ((1, 1, 1), 0), ((3, 5, 7), 0), ((2, 8, 16), 1)])
 def test_fp32_viscous_tiles_are_shape_invariant(shape, wg):
     """The FP32 form is held to a tolerance against FP64, but against itself
