@@ -604,6 +604,59 @@ def sod_c1_full(M) -> dict:
                        "physics": "inviscid Roe, 1st order, Extrapolate on every side", "dt": o.dt}}
 
 
+def plain_fp64_on_cases(M, cases, timed_steps: int = 20, graph: int = 10) -> dict:
+    """The plain FP64 path on a sample of a workload's own cases, one case at
+    a time, as the reference runs them (one case per process).  Each case is
+    built alone in FP64 with the default tiles and stepped in graphs of
+    `graph` steps; its step time is measured over `timed_steps` steps after
+    one untimed graph.  The aggregate is the sample's work at each case's own
+    step count over the summed stepping time.  Mesh generation and setup are
+    left out, which favours the plain path."""
+    from miniaero_amd.mesh import generate_meshes
+    plain = [o.replace(precision=0, autotune=0) for o in cases]
+    meshes = generate_meshes(plain)
+    work = secs = 0.0
+    rates = []
+    for md, o in zip(meshes, plain):
+        s = M.TimeSolverExplicitRK4(md, o)
+        try:
+            s.set_graph(graph)
+            s.initialize()
+            s.step(graph)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.step(timed_steps)
+            s.synchronize()
+            t = (time.perf_counter() - t0) / timed_steps
+            if s.nonfinite():
+                raise RuntimeError("non-finite state in the plain-path sample")
+        finally:
+            s.close()
+        cells = o.nx * o.ny * o.nz
+        work += cells * 4.0 * o.ntimesteps
+        secs += t * o.ntimesteps
+        rates.append(cells * 4.0 / t)
+    cells = [o.nx * o.ny * o.nz for o in plain]
+    return {"value": work / secs, "unit": "cell-stage updates/s", "cases": len(plain),
+            "cells_per_case": {"min": min(cells), "max": max(cells)},
+            "per_case_rate": {"min": min(rates), "max": max(rates)},
+            "method": f"each sampled case alone in FP64 (default tiles, graphs of {graph} steps), step time "
+                      f"over {timed_steps} steps after one untimed graph; aggregate = sample work at each "
+                      f"case's own step count / summed stepping time; setup excluded"}
+
+
+def vs_plain(M, r, cases, n: int = 20) -> None:
+    """Adds the plain FP64 path on about n of the cases (an even stride over
+    the list) and the ratio to r's extra fields."""
+    stride = max(1, len(cases) // n)
+    p = plain_fp64_on_cases(M, cases[::stride])
+    p["sample"] = f"every {stride}th case"
+    r.setdefault("extra", {}).update(plain_fp64_same_cases=p,
+                                     vs_plain_fp64_same_cases=round(r["value"] / p["value"], 3))
+    print(f"[bench] plain FP64 path on {p['cases']} of the same cases: {p['value']:.3e} cell-stage updates/s",
+          file=sys.stderr, flush=True)
+
+
 def bench_suite(args, M, copy_gbs) -> dict:
     """The primary line (FP64 viscous plate, 2.1M cells) plus every other
     headline figure of the path, each under workloads[name]."""
@@ -645,11 +698,21 @@ def bench_suite(args, M, copy_gbs) -> dict:
         x["cpu_baseline"] = same_physics
         return x
 
+    def against_plain(x, cases, sample):
+        # the target's comparison: the plain FP64 path on the same cases
+        p = plain_fp64_on_cases(M, cases[::sample])
+        p["sample"] = f"every {sample}th case"
+        x["plain_fp64_same_cases"] = p
+        x["vs_plain_fp64_same_cases"] = round(x["value"] / p["value"], 3)
+        note(f"plain FP64 path on {p['cases']} of the same cases: {p['value']:.3e} cell-stage updates/s")
+
     def batched_c4():
         # the mixed-precision form, with every case's error against its FP64 run after its 100 steps
         a = _sub(args, workload="batched_c4", graph=2, compare_fp64=True)
         cases = [o.replace(precision=1) for o in BATCH_WORKLOADS["batched_c4"]()]
-        return _compact(bench_batch(a, M, cases, copy_gbs), a.steps)
+        x = _compact(bench_batch(a, M, cases, copy_gbs), a.steps)
+        against_plain(x, cases, 16)
+        return x
 
     def served_200():
         a = _sub(args, workload="served_c5", cases=200, compare_fp64=False)
@@ -658,6 +721,7 @@ def bench_suite(args, M, copy_gbs) -> dict:
         if not args.no_cpu_baseline:
             small = [o for _, o in stream if o.nx * o.ny * o.nz <= 300_000]
             x["cpu_baseline"] = cpu_baseline_batch(small, min_cells=300_000, steps=2)
+        against_plain(x, [o for _, o in stream], 10)
         return x
 
     attempt("flat_plate_fp64_8M", fp64_large)
@@ -701,6 +765,9 @@ def main() -> int:
                     help="served workload: RK4 steps queued ahead per scheme group (0 = the server's default, 2)")
     ap.add_argument("--wg-per-cu", type=int, default=None,
                     help="resident workgroups per CU for the flux kernel (0: one per tile)")
+    ap.add_argument("--vs-plain", action="store_true",
+                    help="batched and served workloads: also time the plain FP64 path on a sample of the "
+                         "same cases, one at a time, and report the ratio")
     ap.add_argument("--copy-ceiling", action="store_true",
                     help="only measure the plain copy kernel (1 GiB, read + write) and exit")
     ap.add_argument("--cpu-worker", action="store_true", help=argparse.SUPPRESS)
@@ -729,7 +796,10 @@ def main() -> int:
         if args.workload == "suite":
             r = bench_suite(args, M, copy_gbs)
         elif args.workload in SERVED_WORKLOADS:
-            r = bench_served(args, M, SERVED_WORKLOADS[args.workload](args.cases), copy_gbs)
+            stream = SERVED_WORKLOADS[args.workload](args.cases)
+            r = bench_served(args, M, stream, copy_gbs)
+            if args.vs_plain:
+                vs_plain(M, r, [o for _, o in stream])
         elif args.workload in BATCH_WORKLOADS:
             cases = BATCH_WORKLOADS[args.workload]()
             if args.precision is not None:
@@ -739,6 +809,8 @@ def main() -> int:
             if args.wg_per_cu is not None:
                 cases = [o.replace(workgroups_per_cu=args.wg_per_cu) for o in cases]
             r = bench_batch(args, M, cases, copy_gbs)
+            if args.vs_plain:
+                vs_plain(M, r, cases)
         else:
             opts = WORKLOADS[args.workload]()
             if args.precision is not None:
