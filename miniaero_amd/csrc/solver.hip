@@ -1034,9 +1034,21 @@ __device__ __forceinline__ void dt_case(const double* __restrict__ partial, cons
                                         double* __restrict__ time_dev, int k, double* red) {
   const double cf = cfl[k];
   double x = 1e300;
-  if (cf > 0.0)
-    for (int i = case_tiles[k] + threadIdx.x; i < case_tiles[k + 1]; i += blockDim.x)
-      x = ma::smin(x, partial[i]);
+  if (cf > 0.0) {
+    // eight loads in flight per thread (a case can have tens of thousands of
+    // tiles); the minimum of positive finite bounds does not depend on the
+    // order it is taken in, and the order is fixed in any case
+    const int hi = case_tiles[k + 1], n = blockDim.x;
+    int i = case_tiles[k] + threadIdx.x;
+    for (; i + 7 * n < hi; i += 8 * n) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = partial[i + u * n];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) x = ma::smin(x, v[u]);
+    }
+    for (; i < hi; i += n) x = ma::smin(x, partial[i]);
+  }
   for (int off = 32; off > 0; off >>= 1) x = ma::smin(x, __shfl_down(x, off, 64));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) red[wave] = x;
@@ -2681,7 +2693,10 @@ ma_status run_persistent(ma_solver* s, int nsteps) {
                                         "workgroups of a cooperative launch from being resident together");
   MA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kTile, lds));
   const int nc = s->m.nc;
-  const int most = std::max(std::max(s->t.ntiles, nblocks(nc)), std::max(nblocks(s->m.nf - s->m.nif), s->ncases));
+  // one workgroup per cell block at most: smaller flux tiles (4x4x4 on small
+  // runs) are walked several per workgroup rather than widening the grid
+  // barriers (16k-cell Sod: 263 us per step with a workgroup per tile)
+  const int most = std::max(nblocks(nc), std::max(nblocks(s->m.nf - s->m.nif), s->ncases));
   int grid = std::min(per_cu * s->cus, (most + kXcds - 1) / kXcds * kXcds);
   grid -= grid % kXcds;
   if (grid < kXcds || wall_khz <= 0)
