@@ -699,8 +699,13 @@ def bench_suite(args, M, copy_gbs) -> dict:
         return x
 
     def against_plain(x, cases, sample):
-        # the target's comparison: the plain FP64 path on the same cases
-        p = plain_fp64_on_cases(M, cases[::sample])
+        # the target's comparison: the plain FP64 path on the same cases (a
+        # failure here is reported without losing the workload's own figures)
+        try:
+            p = plain_fp64_on_cases(M, cases[::sample])
+        except Exception as e:
+            x["plain_fp64_same_cases"] = {"error": f"{type(e).__name__}: {e}"}
+            return
         p["sample"] = f"every {sample}th case"
         x["plain_fp64_same_cases"] = p
         x["vs_plain_fp64_same_cases"] = round(x["value"] / p["value"], 3)
