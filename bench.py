@@ -770,9 +770,9 @@ def main() -> int:
                     help="served workload: RK4 steps queued ahead per scheme group (0 = the server's default, 2)")
     ap.add_argument("--wg-per-cu", type=int, default=None,
                     help="resident workgroups per CU for the flux kernel (0: one per tile)")
-    ap.add_argument("--vs-plain", action="store_true",
-                    help="batched and served workloads: also time the plain FP64 path on a sample of the "
-                         "same cases, one at a time, and report the ratio")
+    ap.add_argument("--vs-plain", type=int, nargs="?", const=20, default=0, metavar="N",
+                    help="batched and served workloads: also time the plain FP64 path on about N (default "
+                         "20) of the same cases, one at a time, and report the ratio")
     ap.add_argument("--copy-ceiling", action="store_true",
                     help="only measure the plain copy kernel (1 GiB, read + write) and exit")
     ap.add_argument("--cpu-worker", action="store_true", help=argparse.SUPPRESS)
@@ -804,7 +804,7 @@ def main() -> int:
             stream = SERVED_WORKLOADS[args.workload](args.cases)
             r = bench_served(args, M, stream, copy_gbs)
             if args.vs_plain:
-                vs_plain(M, r, [o for _, o in stream])
+                vs_plain(M, r, [o for _, o in stream], args.vs_plain)
         elif args.workload in BATCH_WORKLOADS:
             cases = BATCH_WORKLOADS[args.workload]()
             if args.precision is not None:
@@ -815,7 +815,7 @@ def main() -> int:
                 cases = [o.replace(workgroups_per_cu=args.wg_per_cu) for o in cases]
             r = bench_batch(args, M, cases, copy_gbs)
             if args.vs_plain:
-                vs_plain(M, r, cases)
+                vs_plain(M, r, cases, args.vs_plain)
         else:
             opts = WORKLOADS[args.workload]()
             if args.precision is not None:
