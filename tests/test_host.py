@@ -330,6 +330,54 @@ def test_bench_algorithmic_bytes_match_the_stated_figures():
     assert abs(bench.path_bytes_per_step(nc, nif, nbf, True, True) / (4 * nc) - 1333) < 1
 
 
+def test_bench_plain_path_aggregate_weights_each_case_by_its_steps(monkeypatch):
+    """The plain-path comparison beside the batched and served figures: each
+    sampled case runs alone, its step time is taken over the timed steps
+    only (setup and the untimed graph excluded), and the aggregate is the
+    sample's work at each case's own step count over the summed stepping
+    time -- here with a fake solver on a fake clock, two cases whose step
+    times and step counts differ."""
+    import sys
+    sys.path.insert(0, ROOT)
+    import bench
+
+    clock = [0.0]
+    step_s = {64: 1e-3, 512: 2e-3}            # seconds per step, by nx
+    monkeypatch.setattr(bench.time, "perf_counter", lambda: clock[0])
+    import miniaero_amd.mesh as mesh_mod
+    monkeypatch.setattr(mesh_mod, "generate_meshes", lambda opts: [o.nx for o in opts])
+
+    class FakeSolver:
+        def __init__(self, md, o):
+            assert o.precision == 0 and o.autotune == 0
+            self.dt = step_s[md]
+            clock[0] += 100.0                  # setup: must not count
+        def set_graph(self, n):
+            assert n == 10
+        def initialize(self):
+            clock[0] += 50.0
+        def step(self, n):
+            clock[0] += n * self.dt
+        def synchronize(self):
+            pass
+        def nonfinite(self):
+            return 0
+        def close(self):
+            pass
+
+    class FakeM:
+        TimeSolverExplicitRK4 = FakeSolver
+
+    a = M.Options(nx=64, ny=4, nz=4, ntimesteps=100, precision=1)
+    b = M.Options(nx=512, ny=4, nz=4, ntimesteps=300, precision=1)
+    r = bench.plain_fp64_on_cases(FakeM, [a, b], timed_steps=20, graph=10)
+    work = 64 * 16 * 4.0 * 100 + 512 * 16 * 4.0 * 300
+    secs = 1e-3 * 100 + 2e-3 * 300
+    assert r["cases"] == 2
+    assert abs(r["value"] - work / secs) < 1e-6 * work / secs
+    assert abs(r["per_case_rate"]["min"] - 64 * 16 * 4 / 1e-3) < 1e-3
+
+
 def test_axis_aligned_face_frame_equals_general_path(tmp_path):
     """The kernels build the Roe face frame of an axis-aligned area vector with
     selects (axis_frame_t, physics.hpp); it must equal the general path's
