@@ -673,8 +673,9 @@ __device__ __forceinline__ void grad_cell(const DevMesh& m, const int* __restric
 // allow 3 (then 2 waves run); other forms no constraint (the FP32 viscous
 // kernel runs 5 waves at 84 registers; a 6-wave budget spills and was 14%
 // slower on the 512x256x64 plate).  The axis-aligned FP64 viscous kernel
-// spills 12 bytes per lane at 3 waves and is still 10% faster than at 2
-// waves without spills (0.222 against 0.245 ms on the 256x128x64 plate).
+// fits 3 waves in 158 registers with its empty-slot masks folded; before
+// that it spilled 12 bytes per lane and was still 10% faster than a 2-wave
+// build without spills (0.222 against 0.245 ms on the 256x128x64 plate).
 template <bool kSecondOrder, class T>
 constexpr int kGradWaves = sizeof(T) == 8 && kSecondOrder ? 3 : 1;
 
